@@ -1,0 +1,150 @@
+/*
+ * warpcore_c.h -- C ABI of the MI355X-native warpcore hash-table hot path.
+ *
+ * The reference (sleeepyjack/warpcore) is a header-only CUDA C++ library and
+ * has no C ABI; its boundary is the host half of each class template. Each
+ * entry point below replaces one host member function of one explicit
+ * instantiation, with the same argument meaning and the same error model:
+ * nothing throws, operations are asynchronous on `stream`, per-key outcomes
+ * go to the optional `status_out` (one Status word per key, bit layout of
+ * include/warpcore/status.cuh:44-68) and table-wide flags accumulate in the
+ * table's status word (peek_status / pop_status).
+ *
+ * Conventions
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *   - key/value/status/flag buffers are device pointers owned by the caller.
+ *   - functions returning int return 0 on success, a hipError_t of the
+ *     launch otherwise, or -1 for a NULL handle.
+ *   - `probing_length` = WC_DEFAULT_PROBING_LENGTH reproduces the default
+ *     argument of the reference (include/warpcore/defaults.cuh:27).
+ *   - `cg_size` selects the cooperative group size (1..64 lanes of a 64-lane
+ *     wavefront); 0 selects the reference's default for that class.
+ */
+#ifndef WARPCORE_C_H
+#define WARPCORE_C_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+#define WC_API extern "C"
+#else
+#define WC_API
+#endif
+
+#define WC_DEFAULT_PROBING_LENGTH (~(uint64_t)0 - 1024)
+#define WC_DEFAULT_SEED 0x5ad0ded
+
+/* Status flag bits (include/warpcore/status.cuh:44-68). */
+#define WC_STATUS_NONE 0u
+#define WC_STATUS_UNKNOWN_ERROR (1u << 0)
+#define WC_STATUS_PROBING_LENGTH_EXCEEDED (1u << 1)
+#define WC_STATUS_INVALID_CONFIGURATION (1u << 2)
+#define WC_STATUS_INVALID_KEY (1u << 3)
+#define WC_STATUS_DUPLICATE_KEY (1u << 4)
+#define WC_STATUS_KEY_NOT_FOUND (1u << 5)
+#define WC_STATUS_INDEX_OVERFLOW (1u << 6)
+#define WC_STATUS_OUT_OF_MEMORY (1u << 7)
+#define WC_STATUS_NOT_INITIALIZED (1u << 8)
+#define WC_STATUS_DRY_RUN (1u << 9)
+#define WC_STATUS_INVALID_PHASE_OVERLAP (1u << 10)
+#define WC_STATUS_MAX_VALUES_FOR_KEY_REACHED (1u << 11)
+#define WC_STATUS_INVALID_VALUE (1u << 12)
+
+/* ------------------------------------------------------------------------
+ * SingleValueHashTable<K, V> (include/warpcore/single_value_hash_table.cuh)
+ *   _create            ctor (min_capacity, seed, no_init)          :111-128
+ *   _init / _init_seed init(stream) / init(seed, stream)           :175-201
+ *   _insert            insert<ReturnStatus or ReturnNothing>(...)  :239-258
+ *   _insert_bool       insert<ReturnBoolean<>>(...)                :239-258
+ *   _retrieve          retrieve<ReturnStatus or ReturnNothing>     :318-337
+ *   _retrieve_bool     retrieve<ReturnBoolean<>>                   :318-337
+ *   _erase             erase(...)                                  :478-496
+ *   _retrieve_all      retrieve_all(keys_out, values_out, num_out) :390-417
+ *   _size _load_factor _capacity _bytes_total                     :534-630
+ *   _peek_status _pop_status _is_initialized                       :635-687
+ *   _cg_size           cg_size()                                   :100-104
+ * u32u32 and u64u64 accept cg_size 1..64; u32u64 and u64u32 use 8.
+ * ---------------------------------------------------------------------- */
+/* SingleValueHashTable<uint32_t, uint32_t> */
+WC_API void* wc_svht_u32u32_create(uint64_t min_capacity, uint32_t seed, int no_init, uint32_t cg_size);
+WC_API void wc_svht_u32u32_destroy(void* table);
+WC_API int wc_svht_u32u32_init(void* table, void* stream);
+WC_API int wc_svht_u32u32_init_seed(void* table, uint32_t seed, void* stream);
+WC_API int wc_svht_u32u32_insert(void* table, const uint32_t* keys_in, const uint32_t* values_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_svht_u32u32_insert_bool(void* table, const uint32_t* keys_in, const uint32_t* values_in, uint64_t num_in, void* stream, uint64_t probing_length, bool* flags_out);
+WC_API int wc_svht_u32u32_retrieve(void* table, const uint32_t* keys_in, uint64_t num_in, uint32_t* values_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_svht_u32u32_retrieve_bool(void* table, const uint32_t* keys_in, uint64_t num_in, uint32_t* values_out, void* stream, uint64_t probing_length, bool* flags_out);
+WC_API int wc_svht_u32u32_erase(void* table, const uint32_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_svht_u32u32_retrieve_all(void* table, uint32_t* keys_out, uint32_t* values_out, uint64_t* num_out, void* stream);
+WC_API uint64_t wc_svht_u32u32_size(void* table, void* stream);
+WC_API float wc_svht_u32u32_load_factor(void* table, void* stream);
+WC_API uint64_t wc_svht_u32u32_capacity(void* table);
+WC_API uint64_t wc_svht_u32u32_bytes_total(void* table);
+WC_API uint32_t wc_svht_u32u32_peek_status(void* table, void* stream);
+WC_API uint32_t wc_svht_u32u32_pop_status(void* table, void* stream);
+WC_API int wc_svht_u32u32_is_initialized(void* table);
+WC_API uint32_t wc_svht_u32u32_cg_size(void* table);
+
+/* SingleValueHashTable<uint64_t, uint64_t> */
+WC_API void* wc_svht_u64u64_create(uint64_t min_capacity, uint64_t seed, int no_init, uint32_t cg_size);
+WC_API void wc_svht_u64u64_destroy(void* table);
+WC_API int wc_svht_u64u64_init(void* table, void* stream);
+WC_API int wc_svht_u64u64_init_seed(void* table, uint64_t seed, void* stream);
+WC_API int wc_svht_u64u64_insert(void* table, const uint64_t* keys_in, const uint64_t* values_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_svht_u64u64_insert_bool(void* table, const uint64_t* keys_in, const uint64_t* values_in, uint64_t num_in, void* stream, uint64_t probing_length, bool* flags_out);
+WC_API int wc_svht_u64u64_retrieve(void* table, const uint64_t* keys_in, uint64_t num_in, uint64_t* values_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_svht_u64u64_retrieve_bool(void* table, const uint64_t* keys_in, uint64_t num_in, uint64_t* values_out, void* stream, uint64_t probing_length, bool* flags_out);
+WC_API int wc_svht_u64u64_erase(void* table, const uint64_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_svht_u64u64_retrieve_all(void* table, uint64_t* keys_out, uint64_t* values_out, uint64_t* num_out, void* stream);
+WC_API uint64_t wc_svht_u64u64_size(void* table, void* stream);
+WC_API float wc_svht_u64u64_load_factor(void* table, void* stream);
+WC_API uint64_t wc_svht_u64u64_capacity(void* table);
+WC_API uint64_t wc_svht_u64u64_bytes_total(void* table);
+WC_API uint32_t wc_svht_u64u64_peek_status(void* table, void* stream);
+WC_API uint32_t wc_svht_u64u64_pop_status(void* table, void* stream);
+WC_API int wc_svht_u64u64_is_initialized(void* table);
+WC_API uint32_t wc_svht_u64u64_cg_size(void* table);
+
+/* SingleValueHashTable<uint32_t, uint64_t> */
+WC_API void* wc_svht_u32u64_create(uint64_t min_capacity, uint32_t seed, int no_init, uint32_t cg_size);
+WC_API void wc_svht_u32u64_destroy(void* table);
+WC_API int wc_svht_u32u64_init(void* table, void* stream);
+WC_API int wc_svht_u32u64_init_seed(void* table, uint32_t seed, void* stream);
+WC_API int wc_svht_u32u64_insert(void* table, const uint32_t* keys_in, const uint64_t* values_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_svht_u32u64_insert_bool(void* table, const uint32_t* keys_in, const uint64_t* values_in, uint64_t num_in, void* stream, uint64_t probing_length, bool* flags_out);
+WC_API int wc_svht_u32u64_retrieve(void* table, const uint32_t* keys_in, uint64_t num_in, uint64_t* values_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_svht_u32u64_retrieve_bool(void* table, const uint32_t* keys_in, uint64_t num_in, uint64_t* values_out, void* stream, uint64_t probing_length, bool* flags_out);
+WC_API int wc_svht_u32u64_erase(void* table, const uint32_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_svht_u32u64_retrieve_all(void* table, uint32_t* keys_out, uint64_t* values_out, uint64_t* num_out, void* stream);
+WC_API uint64_t wc_svht_u32u64_size(void* table, void* stream);
+WC_API float wc_svht_u32u64_load_factor(void* table, void* stream);
+WC_API uint64_t wc_svht_u32u64_capacity(void* table);
+WC_API uint64_t wc_svht_u32u64_bytes_total(void* table);
+WC_API uint32_t wc_svht_u32u64_peek_status(void* table, void* stream);
+WC_API uint32_t wc_svht_u32u64_pop_status(void* table, void* stream);
+WC_API int wc_svht_u32u64_is_initialized(void* table);
+WC_API uint32_t wc_svht_u32u64_cg_size(void* table);
+
+/* SingleValueHashTable<uint64_t, uint32_t> */
+WC_API void* wc_svht_u64u32_create(uint64_t min_capacity, uint64_t seed, int no_init, uint32_t cg_size);
+WC_API void wc_svht_u64u32_destroy(void* table);
+WC_API int wc_svht_u64u32_init(void* table, void* stream);
+WC_API int wc_svht_u64u32_init_seed(void* table, uint64_t seed, void* stream);
+WC_API int wc_svht_u64u32_insert(void* table, const uint64_t* keys_in, const uint32_t* values_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_svht_u64u32_insert_bool(void* table, const uint64_t* keys_in, const uint32_t* values_in, uint64_t num_in, void* stream, uint64_t probing_length, bool* flags_out);
+WC_API int wc_svht_u64u32_retrieve(void* table, const uint64_t* keys_in, uint64_t num_in, uint32_t* values_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_svht_u64u32_retrieve_bool(void* table, const uint64_t* keys_in, uint64_t num_in, uint32_t* values_out, void* stream, uint64_t probing_length, bool* flags_out);
+WC_API int wc_svht_u64u32_erase(void* table, const uint64_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_svht_u64u32_retrieve_all(void* table, uint64_t* keys_out, uint32_t* values_out, uint64_t* num_out, void* stream);
+WC_API uint64_t wc_svht_u64u32_size(void* table, void* stream);
+WC_API float wc_svht_u64u32_load_factor(void* table, void* stream);
+WC_API uint64_t wc_svht_u64u32_capacity(void* table);
+WC_API uint64_t wc_svht_u64u32_bytes_total(void* table);
+WC_API uint32_t wc_svht_u64u32_peek_status(void* table, void* stream);
+WC_API uint32_t wc_svht_u64u32_pop_status(void* table, void* stream);
+WC_API int wc_svht_u64u32_is_initialized(void* table);
+WC_API uint32_t wc_svht_u64u32_cg_size(void* table);
+
+
+#endif /* WARPCORE_C_H */

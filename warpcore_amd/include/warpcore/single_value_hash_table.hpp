@@ -1,0 +1,424 @@
+// SingleValueHashTable: key -> one value, open addressing over line-aligned
+// windows. Drop-in for include/warpcore/single_value_hash_table.cuh: same
+// template parameters, host calls (insert / retrieve / erase / retrieve_all /
+// for_each / size / load_factor / peek_status / pop_status ...) and device
+// calls taking a tile<CG> (the reference's thread_block_tile<CG>).
+//
+// Insert protocol per probe window (single_value_hash_table.cuh:742-823):
+// every lane reads one slot; a hit anywhere in the window means duplicate
+// (the value is overwritten); otherwise the lowest EMPTY lane tries a
+// compare-and-swap, the window's other EMPTY lanes are tried in order, and a
+// CAS that returns the key itself is a duplicate. Tombstones are never
+// reused. For 8-byte u32/u32 slots the CAS swaps key and value together, so
+// a reader never sees the key without its value; wider slots publish the
+// key first and store the value after it, as the reference does.
+#pragma once
+
+#include "base.hpp"
+#include "capacity.hpp"
+#include "defaults.hpp"
+#include "engine.hpp"
+#include "probing_schemes.hpp"
+#include "status.hpp"
+#include "storage.hpp"
+#include "tile.hpp"
+
+namespace warpcore {
+
+template<class Key, class Value, Key EmptyKey, Key TombstoneKey, class ProbingScheme, class TableStorage,
+         index_t TempMemoryBytes>
+class CountingHashTable;
+
+template<class Key, class Value, Key EmptyKey, Key TombstoneKey, class ValueStore, class ProbingScheme>
+class BucketListHashTable;
+
+namespace detail {
+
+// Trivially copyable device state of a single-value table; batch kernels
+// receive this by value.
+template<class Key, class Value, Key EmptyKey, Key TombstoneKey, class ProbingScheme, class Slot>
+struct SingleValueView {
+    using key_type = Key;
+    using value_type = Value;
+    using slot_type = Slot;
+    using index_type = index_t;
+    static constexpr bool packed = is_packed_pair<Slot>();
+
+    Slot* slots;
+    Status* status;
+    probing_schemes::Geometry geo;
+    Key seed;
+    bool initialized;
+
+    WC_HD static constexpr index_type cg_size() noexcept { return ProbingScheme::cg_size(); }
+    WC_HD static constexpr bool is_empty_key(Key k) noexcept { return k == EmptyKey; }
+    WC_HD static constexpr bool is_tombstone_key(Key k) noexcept { return k == TombstoneKey; }
+    WC_HD static constexpr bool is_valid_key(Key k) noexcept { return k != EmptyKey && k != TombstoneKey; }
+    WC_HD index_type capacity() const noexcept { return geo.capacity; }
+    WC_HD Status* status_word() const noexcept { return status; }
+    WC_D Key key_at(index_type i) const noexcept { return slots[i].key; }
+    WC_D Slot slot_at(index_type i) const noexcept { return load_slot(slots + i); }
+
+    // Finds or claims the slot of `key_in`. Returns the slot index, or ~0 if
+    // none (invalid key, probing exhausted). With `publish` and a packed
+    // slot the claiming CAS also writes *publish as the value, and
+    // `published` is set.
+    template<class Join>
+    WC_D index_type insert_impl(Key key_in, Status& status_out, const tile<cg_size()>& group,
+                                index_type probing_length, Join& join, const Value* publish,
+                                bool& published) const noexcept {
+        published = false;
+        if (!initialized) { status_out = Status::not_initialized(); return ~index_type(0); }
+        if (!is_valid_key(key_in)) {
+            join(Status::invalid_key());
+            status_out = Status::invalid_key();
+            return ~index_type(0);
+        }
+        ProbingScheme iter(geo, probing_length, group);
+        for (index_type i = iter.begin(key_in, seed); i != iter.end(); i = iter.next()) {
+            const Slot s = load_slot(slots + i);
+            const Key table_key = s.key;
+            const auto hit_mask = group.ballot(table_key == key_in);
+            if (hit_mask) {
+                join(Status::duplicate_key());
+                status_out = Status::duplicate_key();
+                return group.shfl(i, unsigned(ffs64(hit_mask) - 1));
+            }
+            auto empty_mask = group.ballot(is_empty_key(table_key));
+            while (empty_mask) {
+                const unsigned leader = unsigned(ffs64(empty_mask) - 1);
+                bool success = false;
+                bool duplicate = false;
+                if (group.thread_rank() == leader) {
+                    if constexpr (packed) {
+                        std::uint64_t expected = slot_bits(s);
+                        if (publish != nullptr) {
+                            Slot want;
+                            want.key = key_in;
+                            want.value = *publish;
+                            const std::uint64_t desired = slot_bits(want);
+                            while (true) {
+                                const std::uint64_t old =
+                                    atomic_cas(reinterpret_cast<std::uint64_t*>(slots + i), expected, desired);
+                                if (old == expected) { success = true; break; }
+                                const Key old_key = slot_from_bits<Slot>(old).key;
+                                if (old_key != EmptyKey) { duplicate = old_key == key_in; break; }
+                                expected = old;  // value bits of an EMPTY slot moved; retry
+                            }
+                            published = success;
+                        } else {
+                            const Key old = atomic_cas(&(slots[i].key), table_key, key_in);
+                            success = old == table_key;
+                            duplicate = old == key_in;
+                        }
+                    } else {
+                        const Key old = atomic_cas(&(slots[i].key), table_key, key_in);
+                        success = old == table_key;
+                        duplicate = old == key_in;
+                    }
+                }
+                if (group.any(duplicate)) {
+                    join(Status::duplicate_key());
+                    status_out = Status::duplicate_key();
+                    return group.shfl(i, leader);
+                }
+                if (group.any(success)) {
+                    status_out = Status::none();
+                    published = group.any(published);
+                    return group.shfl(i, leader);
+                }
+                empty_mask ^= std::uint64_t(1) << leader;
+            }
+        }
+        join(Status::probing_length_exceeded());
+        status_out = Status::probing_length_exceeded();
+        return ~index_type(0);
+    }
+
+    // single_value_hash_table.cuh:210-228
+    template<class Join>
+    WC_D Status insert(Key key_in, const Value& value_in, const tile<cg_size()>& group, index_type probing_length,
+                       Join& join) const noexcept {
+        Status status = Status::unknown_error();
+        bool published = false;
+        const index_type s = insert_impl(key_in, status, group, probing_length, join, &value_in, published);
+        if (group.thread_rank() == 0 && s != ~index_type(0) && !published) slots[s].value = value_in;
+        return status;
+    }
+
+    // single_value_hash_table.cuh:267-307; the value comes from the same
+    // slot read that found the key
+    template<class Join>
+    WC_D Status retrieve(Key key_in, Value& value_out, const tile<cg_size()>& group, index_type probing_length,
+                         Join& join) const noexcept {
+        if (!initialized) return Status::not_initialized();
+        if (!is_valid_key(key_in)) { join(Status::invalid_key()); return Status::invalid_key(); }
+        ProbingScheme iter(geo, probing_length, group);
+        for (index_type i = iter.begin(key_in, seed); i != iter.end(); i = iter.next()) {
+            const Slot s = load_slot(slots + i);
+            const auto hit_mask = group.ballot(s.key == key_in);
+            if (hit_mask) {
+                value_out = group.shfl(s.value, unsigned(ffs64(hit_mask) - 1));
+                return Status::none();
+            }
+            if (group.any(is_empty_key(s.key))) {
+                join(Status::key_not_found());
+                return Status::key_not_found();
+            }
+        }
+        join(Status::probing_length_exceeded());
+        return Status::probing_length_exceeded();
+    }
+
+    // single_value_hash_table.cuh:425-468; misses are returned, not joined
+    template<class Join>
+    WC_D Status erase(Key key_in, const tile<cg_size()>& group, index_type probing_length, Join& join) const noexcept {
+        if (!initialized) return Status::not_initialized();
+        if (!is_valid_key(key_in)) { join(Status::invalid_key()); return Status::invalid_key(); }
+        ProbingScheme iter(geo, probing_length, group);
+        for (index_type i = iter.begin(key_in, seed); i != iter.end(); i = iter.next()) {
+            const Key table_key = slots[i].key;
+            const auto hit_mask = group.ballot(table_key == key_in);
+            if (hit_mask) {
+                if (group.thread_rank() == unsigned(ffs64(hit_mask) - 1)) slots[i].key = TombstoneKey;
+                return Status::none();
+            }
+            if (group.any(is_empty_key(table_key))) return Status::key_not_found();
+        }
+        return Status::probing_length_exceeded();
+    }
+};
+
+}  // namespace detail
+
+template<class Key, class Value, Key EmptyKey = defaults::empty_key<Key>(),
+         Key TombstoneKey = defaults::tombstone_key<Key>(),
+         class ProbingScheme = defaults::probing_scheme_t<Key, 8>,
+         class TableStorage = defaults::table_storage_t<Key, Value>,
+         index_t TempMemoryBytes = defaults::temp_memory_bytes()>
+class SingleValueHashTable {
+    static_assert(checks::is_valid_key_type<Key>(), "invalid key type");
+    static_assert(EmptyKey != TombstoneKey, "empty key and tombstone key must not be identical");
+    static_assert(checks::is_probing_scheme<ProbingScheme>(), "not a valid probing scheme type");
+    static_assert(std::is_same<typename ProbingScheme::key_type, Key>::value,
+                  "probing key type differs from table's key type");
+    static_assert(checks::is_key_value_storage<TableStorage>(), "not a valid storage type");
+    static_assert(std::is_same<typename TableStorage::key_type, Key>::value,
+                  "storage's key type differs from table's key type");
+    static_assert(std::is_same<typename TableStorage::value_type, Value>::value,
+                  "storage's value type differs from table's value type");
+    static_assert(TempMemoryBytes >= sizeof(index_t), "temporary storage must at least be of size index_type");
+
+    using temp_type = storage::CyclicStore<index_t>;
+
+public:
+    using key_type = Key;
+    using value_type = Value;
+    using index_type = index_t;
+    using status_type = Status;
+    using probing_scheme_type = ProbingScheme;
+    using slot_type = typename TableStorage::slot_type;
+    using view_type = detail::SingleValueView<Key, Value, EmptyKey, TombstoneKey, ProbingScheme, slot_type>;
+
+    WC_HD static constexpr key_type empty_key() noexcept { return EmptyKey; }
+    WC_HD static constexpr key_type tombstone_key() noexcept { return TombstoneKey; }
+    WC_HD static constexpr index_type cg_size() noexcept { return ProbingScheme::cg_size(); }
+
+    // single_value_hash_table.cuh:111-128
+    WC_H explicit SingleValueHashTable(index_type min_capacity, key_type seed = defaults::seed<key_type>(),
+                                       bool no_init = false) noexcept
+        : status_(nullptr),
+          table_(detail::get_valid_capacity(min_capacity, cg_size())),
+          temp_(TempMemoryBytes / sizeof(index_type)),
+          seed_(seed),
+          is_initialized_(false),
+          is_copy_(false),
+          geo_(probing_schemes::Geometry::make(table_.capacity(), cg_size())) {
+        if (hipMalloc(&status_, sizeof(status_type)) != hipSuccess) status_ = nullptr;
+        assign_status(table_.status() + temp_.status());
+        if (!no_init) init();
+    }
+
+    WC_HD SingleValueHashTable(const SingleValueHashTable& o) noexcept
+        : status_(o.status_), table_(o.table_), temp_(o.temp_), seed_(o.seed_),
+          is_initialized_(o.is_initialized_), is_copy_(true), geo_(o.geo_) {}
+
+    WC_H SingleValueHashTable(SingleValueHashTable&& o) noexcept
+        : status_(o.status_), table_(std::move(o.table_)), temp_(std::move(o.temp_)), seed_(o.seed_),
+          is_initialized_(o.is_initialized_), is_copy_(o.is_copy_), geo_(o.geo_) {
+        o.is_copy_ = true;
+    }
+
+    WC_HD ~SingleValueHashTable() noexcept {
+#if !defined(__HIP_DEVICE_COMPILE__)
+        if (!is_copy_ && status_ != nullptr) (void)hipFree(status_);
+#endif
+    }
+
+    // single_value_hash_table.cuh:175-201
+    WC_H void init(key_type seed, hipStream_t stream = 0) noexcept {
+        is_initialized_ = false;
+        seed_ = seed;
+        if (!table_.status().has_not_initialized() && !temp_.status().has_not_initialized()) {
+            table_.init_keys(empty_key(), stream);
+            assign_status(table_.status() + temp_.status(), stream);
+            is_initialized_ = true;
+        }
+    }
+    WC_H void init(hipStream_t stream = 0) noexcept { init(seed_, stream); }
+
+    // ----------------------------------------------------------- device API
+    WC_D status_type insert(key_type key_in, const value_type& value_in, const tile<cg_size()>& group,
+                            index_type probing_length = defaults::probing_length()) const noexcept {
+        detail::ImmediateJoin join{status_};
+        return view().insert(key_in, value_in, group, probing_length, join);
+    }
+    WC_D status_type retrieve(key_type key_in, value_type& value_out, const tile<cg_size()>& group,
+                              index_type probing_length = defaults::probing_length()) const noexcept {
+        detail::ImmediateJoin join{status_};
+        return view().retrieve(key_in, value_out, group, probing_length, join);
+    }
+    WC_D status_type erase(key_type key_in, const tile<cg_size()>& group,
+                           index_type probing_length = defaults::probing_length()) const noexcept {
+        detail::ImmediateJoin join{status_};
+        return view().erase(key_in, group, probing_length, join);
+    }
+
+    // ------------------------------------------------------------- host API
+    // single_value_hash_table.cuh:239-258
+    template<class StatusHandler = defaults::status_handler_t>
+    WC_H void insert(const key_type* keys_in, const value_type* values_in, index_type num_in, hipStream_t stream = 0,
+                     index_type probing_length = defaults::probing_length(),
+                     typename StatusHandler::base_type* status_out = nullptr) noexcept {
+        static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
+        if (!is_initialized_ || num_in == 0) return;
+        kernels::insert<view_type, StatusHandler>
+            <<<detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(keys_in, values_in, num_in, view(),
+                                                                                probing_length, status_out);
+    }
+
+    // single_value_hash_table.cuh:318-337
+    template<class StatusHandler = defaults::status_handler_t>
+    WC_H void retrieve(const key_type* keys_in, index_type num_in, value_type* values_out, hipStream_t stream = 0,
+                       index_type probing_length = defaults::probing_length(),
+                       typename StatusHandler::base_type* status_out = nullptr) const noexcept {
+        static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
+        if (!is_initialized_ || num_in == 0) return;
+        kernels::retrieve<view_type, StatusHandler>
+            <<<detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(keys_in, num_in, values_out, view(),
+                                                                                probing_length, status_out);
+    }
+
+    // single_value_hash_table.cuh:390-417 (order unspecified)
+    WC_H void retrieve_all(key_type* keys_out, value_type* values_out, index_type& num_out,
+                           hipStream_t stream = 0) const noexcept {
+        if (!is_initialized_) return;
+        index_type* tmp = temp_.get();
+        (void)hipMemsetAsync(tmp, 0, sizeof(index_type), stream);
+        kernels::retrieve_all<view_type>
+            <<<detail::grid_for(capacity(), 1, 8), block_size(), 0, stream>>>(keys_out, values_out, tmp, view());
+        (void)hipMemcpyAsync(&num_out, tmp, sizeof(index_type), hipMemcpyDeviceToHost, stream);
+        (void)hipStreamSynchronize(stream);
+    }
+
+    // single_value_hash_table.cuh:478-496
+    template<class StatusHandler = defaults::status_handler_t>
+    WC_H void erase(const key_type* keys_in, index_type num_in, hipStream_t stream = 0,
+                    index_type probing_length = defaults::probing_length(),
+                    typename StatusHandler::base_type* status_out = nullptr) noexcept {
+        static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
+        if (!is_initialized_ || num_in == 0) return;
+        kernels::erase<view_type, StatusHandler>
+            <<<detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(keys_in, num_in, view(),
+                                                                                probing_length, status_out);
+    }
+
+    // f(key, value) on every stored pair (single_value_hash_table.cuh:504-528)
+    template<class Func>
+    WC_H void for_each(Func f, hipStream_t stream = 0, index_type smem_bytes = 0) const noexcept {
+        if (!is_initialized_) return;
+        kernels::for_each<Func, view_type>
+            <<<detail::grid_for(capacity(), 1, 8), block_size(), smem_bytes, stream>>>(f, view());
+    }
+
+    // single_value_hash_table.cuh:534-592
+    WC_H index_type size(hipStream_t stream = 0) const noexcept {
+        if (!is_initialized_) return 0;
+        index_type out = 0;
+        index_type* tmp = temp_.get();
+        (void)hipMemsetAsync(tmp, 0, sizeof(index_type), stream);
+        kernels::size<view_type><<<detail::grid_for(capacity(), 1, 8), block_size(), 0, stream>>>(tmp, view());
+        (void)hipMemcpyAsync(&out, tmp, sizeof(index_type), hipMemcpyDeviceToHost, stream);
+        (void)hipStreamSynchronize(stream);
+        return out;
+    }
+
+    WC_H float load_factor(hipStream_t stream = 0) const noexcept { return float(size(stream)) / float(capacity()); }
+    WC_H float storage_density(hipStream_t stream = 0) const noexcept { return load_factor(stream); }
+    WC_HD index_type capacity() const noexcept { return table_.capacity(); }
+    WC_H index_type bytes_total() const noexcept {
+        return table_.bytes_total() + temp_.bytes_total() + sizeof(status_type);
+    }
+    WC_HD bool is_initialized() const noexcept { return is_initialized_; }
+
+    // single_value_hash_table.cuh:645-687
+    WC_H status_type peek_status(hipStream_t stream = 0) const noexcept {
+        status_type status = status_type::not_initialized();
+        if (status_ != nullptr) {
+            (void)hipMemcpyAsync(&status, status_, sizeof(status_type), hipMemcpyDeviceToHost, stream);
+            (void)hipStreamSynchronize(stream);
+        }
+        return status;
+    }
+    WC_H status_type pop_status(hipStream_t stream = 0) noexcept {
+        status_type status = status_type::not_initialized();
+        if (status_ != nullptr) {
+            (void)hipMemcpyAsync(&status, status_, sizeof(status_type), hipMemcpyDeviceToHost, stream);
+            assign_status(table_.status(), stream);
+        }
+        return status;
+    }
+
+    WC_HD static constexpr bool is_empty_key(key_type key) noexcept { return key == empty_key(); }
+    WC_HD static constexpr bool is_tombstone_key(key_type key) noexcept { return key == tombstone_key(); }
+    WC_HD static constexpr bool is_valid_key(key_type key) noexcept {
+        return key != empty_key() && key != tombstone_key();
+    }
+    WC_HD key_type seed() const noexcept { return seed_; }
+    WC_HD bool is_copy() const noexcept { return is_copy_; }
+
+    // Device state handed to kernels.
+    WC_HD view_type view() const noexcept {
+        return view_type{table_.data(), status_, geo_, seed_, is_initialized_};
+    }
+
+private:
+    WC_H void assign_status(status_type status, hipStream_t stream = 0) const noexcept {
+        if (status_ != nullptr) {
+            (void)hipMemcpyAsync(status_, &status, sizeof(status_type), hipMemcpyHostToDevice, stream);
+            (void)hipStreamSynchronize(stream);
+        }
+    }
+    WC_H void join_status(status_type status, hipStream_t stream = 0) const noexcept {
+        if (status_ != nullptr) {
+            const status_type joined = peek_status(stream) + status;
+            assign_status(joined, stream);
+        }
+    }
+    WC_D void device_join_status(status_type status) const noexcept {
+        if (status_ != nullptr) status_->atomic_join(status);
+    }
+
+    status_type* status_;
+    TableStorage table_;
+    temp_type temp_;
+    key_type seed_;
+    bool is_initialized_;
+    bool is_copy_;
+    probing_schemes::Geometry geo_;
+
+    template<class K_, class V_, K_ E_, K_ T_, class P_, class S_, index_t B_> friend class CountingHashTable;
+    template<class K_, class V_, K_ E_, K_ T_, class VS_, class P_> friend class BucketListHashTable;
+};
+
+}  // namespace warpcore

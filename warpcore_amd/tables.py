@@ -1,0 +1,160 @@
+"""Python mirror of the reference's host API over the C ABI.
+
+Each class mirrors one class template of sleeepyjack/warpcore for a fixed
+(key, value) instantiation: the same constructor arguments, the same member
+functions with the same argument order (device buffers, stream, probing
+length, optional per-key status output), and the same error model (nothing
+raises for per-key outcomes; flags go to ``status_out`` and to the table's
+status word). Buffers are CUDA tensors (torch on ROCm) of the matching
+element width; signedness is ignored, bits are what count.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _capi
+
+DEFAULT_SEED = 0x5AD0DED
+DEFAULT_PROBING_LENGTH = (1 << 64) - 1 - 1024
+
+
+class Status(int):
+    """Status word (include/warpcore/status.cuh:44-68)."""
+
+    NONE = 0
+    UNKNOWN_ERROR = 1 << 0
+    PROBING_LENGTH_EXCEEDED = 1 << 1
+    INVALID_CONFIGURATION = 1 << 2
+    INVALID_KEY = 1 << 3
+    DUPLICATE_KEY = 1 << 4
+    KEY_NOT_FOUND = 1 << 5
+    INDEX_OVERFLOW = 1 << 6
+    OUT_OF_MEMORY = 1 << 7
+    NOT_INITIALIZED = 1 << 8
+    DRY_RUN = 1 << 9
+    INVALID_PHASE_OVERLAP = 1 << 10
+    MAX_VALUES_FOR_KEY_REACHED = 1 << 11
+    INVALID_VALUE = 1 << 12
+
+    def has_any(self, mask: int = 0x1FFF) -> bool:
+        return bool(self & mask)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("device buffers must be CUDA tensors")
+    if not t.is_contiguous():
+        raise ValueError("device buffers must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    if isinstance(stream, torch.cuda.Stream):
+        return ctypes.c_void_p(stream.cuda_stream)
+    return ctypes.c_void_p(int(stream))
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with code {rc}")
+
+
+def _width(t, bits: int, what: str):
+    if t is not None and t.element_size() * 8 != bits:
+        raise ValueError(f"{what} must have {bits}-bit elements, got {t.dtype}")
+
+
+class SingleValueHashTable:
+    """SingleValueHashTable<K, V> (include/warpcore/single_value_hash_table.cuh)."""
+
+    _NAMES = {(32, 32): "u32u32", (64, 64): "u64u64", (32, 64): "u32u64", (64, 32): "u64u32"}
+
+    def __init__(self, min_capacity: int, key_bits: int = 32, value_bits: int = 32,
+                 seed: int = DEFAULT_SEED, no_init: bool = False, cg_size: int = 0):
+        self.key_bits, self.value_bits = key_bits, value_bits
+        self._p = f"wc_svht_{self._NAMES[(key_bits, value_bits)]}_"
+        self._lib = _capi.lib()
+        self._h = self._fn("create")(min_capacity, seed, int(no_init), cg_size)
+        if not self._h:
+            raise ValueError(f"unsupported configuration (cg_size={cg_size})")
+
+    def _fn(self, name):
+        return getattr(self._lib, self._p + name)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._fn("destroy")(h)
+
+    def init(self, seed: int | None = None, stream=None):
+        if seed is None:
+            _check(self._fn("init")(self._h, _stream(stream)), "init")
+        else:
+            _check(self._fn("init_seed")(self._h, seed, _stream(stream)), "init")
+
+    def insert(self, keys_in, values_in, stream=None, probing_length=DEFAULT_PROBING_LENGTH,
+               status_out=None, num_in: int | None = None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(values_in, self.value_bits, "values_in")
+        _width(status_out, 32, "status_out")
+        n = keys_in.numel() if num_in is None else num_in
+        _check(self._fn("insert")(self._h, _ptr(keys_in), _ptr(values_in), n, _stream(stream),
+                                  probing_length, _ptr(status_out)), "insert")
+
+    def insert_bool(self, keys_in, values_in, flags_out, stream=None, probing_length=DEFAULT_PROBING_LENGTH):
+        _check(self._fn("insert_bool")(self._h, _ptr(keys_in), _ptr(values_in), keys_in.numel(),
+                                       _stream(stream), probing_length, _ptr(flags_out)), "insert")
+
+    def retrieve(self, keys_in, values_out, stream=None, probing_length=DEFAULT_PROBING_LENGTH,
+                 status_out=None, num_in: int | None = None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(values_out, self.value_bits, "values_out")
+        _width(status_out, 32, "status_out")
+        n = keys_in.numel() if num_in is None else num_in
+        _check(self._fn("retrieve")(self._h, _ptr(keys_in), n, _ptr(values_out), _stream(stream),
+                                    probing_length, _ptr(status_out)), "retrieve")
+
+    def retrieve_bool(self, keys_in, values_out, flags_out, stream=None, probing_length=DEFAULT_PROBING_LENGTH):
+        _check(self._fn("retrieve_bool")(self._h, _ptr(keys_in), keys_in.numel(), _ptr(values_out),
+                                         _stream(stream), probing_length, _ptr(flags_out)), "retrieve")
+
+    def erase(self, keys_in, stream=None, probing_length=DEFAULT_PROBING_LENGTH, status_out=None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _check(self._fn("erase")(self._h, _ptr(keys_in), keys_in.numel(), _stream(stream),
+                                 probing_length, _ptr(status_out)), "erase")
+
+    def retrieve_all(self, keys_out, values_out, stream=None) -> int:
+        n = ctypes.c_uint64(0)
+        _check(self._fn("retrieve_all")(self._h, _ptr(keys_out), _ptr(values_out), ctypes.byref(n),
+                                        _stream(stream)), "retrieve_all")
+        return n.value
+
+    def size(self, stream=None) -> int:
+        return self._fn("size")(self._h, _stream(stream))
+
+    def load_factor(self, stream=None) -> float:
+        return self._fn("load_factor")(self._h, _stream(stream))
+
+    def capacity(self) -> int:
+        return self._fn("capacity")(self._h)
+
+    def bytes_total(self) -> int:
+        return self._fn("bytes_total")(self._h)
+
+    def peek_status(self, stream=None) -> Status:
+        return Status(self._fn("peek_status")(self._h, _stream(stream)))
+
+    def pop_status(self, stream=None) -> Status:
+        return Status(self._fn("pop_status")(self._h, _stream(stream)))
+
+    def is_initialized(self) -> bool:
+        return bool(self._fn("is_initialized")(self._h))
+
+    def cg_size(self) -> int:
+        return self._fn("cg_size")(self._h)
