@@ -1,0 +1,312 @@
+#!/usr/bin/env python3
+"""Hash-table hot-path benchmark on one MI355X.
+
+Workload (the large single-value configuration): SingleValueHashTable
+<u64, u64>, 2^28 unique keys at load factor 0.9 (min capacity 2^28 / 0.9 ->
+298,876,376 slots, 4.78 GB of 16-byte slots), cooperative groups of 8 lanes
+(one 128-byte line per probe window), then 2^28 lookups with a 50 % hit
+rate. One step = init (EMPTY fill) + insert of the batch + retrieve of the
+queries; every input is resident in HBM before timing starts.
+
+value = (2^28 inserts + 2^28 lookups) per step / wall time per step, Gops/s.
+Alongside: per-kernel times from HIP events recorded on the stream the
+kernels run on, the roofline of the dominant kernel (algorithmic bytes: a
+retrieve reads one 128-byte line plus key and value, an insert reads and
+writes back one line plus key and value), the same workload for
+SingleValueHashTable<u32, u32> (cg 16) and CountingHashTable<u64> over 2^28
+Zipf(1.1) occurrences of 2^24 keys, and the OpenMP CPU table on a bounded
+sample as the reported CPU baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import math
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+METRIC = "Gops/s insert+retrieve @LF0.9 u32/u64 on 1 GPU; served p99 vs launch-per-request"
+LINE = 128          # bytes per probe window / cache line
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(op: str, kb: int, vb: int) -> int:
+    if op == "retrieve":
+        return LINE + kb // 8 + vb // 8
+    if op == "insert":
+        return 2 * LINE + kb // 8 + vb // 8
+    if op == "count":
+        return 2 * LINE + kb // 8
+    raise ValueError(op)
+
+
+def read_traffic(kernel_tag: str):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy kernel of known size), if a summary for this
+    workload is committed under profiles/."""
+    f = REPO / "profiles" / "svht_u64_lf09_traffic.md"
+    if not f.exists():
+        return None
+    m = re.search(rf"^traffic_bytes_per_launch\.{kernel_tag}\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
+    return float(m.group(1)) if m else None
+
+
+class Gpu:
+    def __init__(self):
+        import torch
+        from warpcore_amd import _capi
+        self.torch = torch
+        self.lib = _capi.lib()
+        self.stream = torch.cuda.Stream()
+        self.sp = ctypes.c_void_p(self.stream.cuda_stream)
+
+    def empty(self, n, bits):
+        t = {32: self.torch.int32, 64: self.torch.int64}[bits]
+        return self.torch.empty(n, dtype=t, device="cuda")
+
+    @staticmethod
+    def p(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    def event(self):
+        return self.torch.cuda.Event(enable_timing=True)
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what} returned {rc}")
+
+
+def run_svht(g: Gpu, kb: int, vb: int, cg: int, log2n: int, steps: int, warmup: int):
+    torch = g.torch
+    from warpcore_amd.tables import SingleValueHashTable
+    n = 1 << log2n
+    half = n // 2
+    min_cap = math.ceil(n / 0.9)
+    sfx = "u64" if kb == 64 else "u32"
+    gen_unique = getattr(g.lib, f"wc_gen_unique_{sfx}")
+    gen_uniform = getattr(g.lib, f"wc_gen_uniform_{'u64' if vb == 64 else 'u32'}")
+    interleave = getattr(g.lib, f"wc_gen_interleave_{sfx}")
+    keys, values = g.empty(n, kb), g.empty(n, vb)
+    absent, queries, out = g.empty(half, kb), g.empty(n, kb), g.empty(n, vb)
+    with torch.cuda.stream(g.stream):
+        g.check(gen_unique(g.p(keys), n, 0x1000, g.sp), "gen keys")
+        g.check(gen_uniform(g.p(values), n, 0xC1 + 1, g.sp), "gen values")
+        g.check(gen_unique(g.p(absent), half, 0x1000 + n + 2, g.sp), "gen absent")
+        g.check(interleave(g.p(queries), g.p(keys), g.p(absent), half, 0xC1 + 2, g.sp), "mix")
+    table = SingleValueHashTable(min_cap, kb, vb, cg_size=cg)
+    g.stream.synchronize()
+
+    def step(ev=None):
+        table.init(stream=g.stream)
+        if ev:
+            ev[0].record(g.stream)
+        table.insert(keys, values, stream=g.stream)
+        if ev:
+            ev[1].record(g.stream)
+        table.retrieve(queries, out, stream=g.stream)
+        if ev:
+            ev[2].record(g.stream)
+
+    for _ in range(warmup):
+        step()
+    g.stream.synchronize()
+    evs = [[g.event() for _ in range(3)] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        step(evs[i])
+    g.stream.synchronize()
+    wall = time.perf_counter() - t0
+    t_ins = [e[0].elapsed_time(e[1]) for e in evs]
+    t_ret = [e[1].elapsed_time(e[2]) for e in evs]
+
+    # result check of the last step: every inserted key stored, half the lookups hit
+    st = g.empty(n, 32)
+    out.fill_(-1)
+    table.retrieve(queries, out, stream=g.stream, status_out=st)
+    g.stream.synchronize()
+    hits = int((st == 0).sum().item())
+    size = table.size(stream=g.stream)
+    status = int(table.peek_status(stream=g.stream))
+    ok = hits == half and size == n and status == 0x20  # key_not_found joined by the misses
+    if not ok:
+        raise RuntimeError(f"result check failed: hits={hits} size={size} status={status:#x}")
+    res = dict(n=n, capacity=table.capacity(), cg=cg, wall_ms_per_step=1e3 * wall / steps,
+               insert_ms=sum(t_ins) / steps, retrieve_ms=sum(t_ret) / steps,
+               insert_ms_min=min(t_ins), retrieve_ms_min=min(t_ret), table_bytes=table.bytes_total())
+    del table, keys, values, absent, queries, out, st
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_counting(g: Gpu, log2n: int, log2_distinct: int, s: float, steps: int, warmup: int):
+    torch = g.torch
+    from warpcore_amd.tables import CountingHashTable
+    n, d = 1 << log2n, 1 << log2_distinct
+    keys = g.empty(n, 64)
+    with torch.cuda.stream(g.stream):
+        g.check(g.lib.wc_gen_zipf_u64(g.p(keys), n, d, s, 0xC3, 0x2000, g.sp), "gen zipf")
+    table = CountingHashTable(math.ceil(d / 0.9), 64)
+    g.stream.synchronize()
+    for _ in range(warmup):
+        table.init(stream=g.stream)
+        table.insert(keys, stream=g.stream)
+    g.stream.synchronize()
+    evs = [[g.event(), g.event()] for _ in range(steps)]
+    t0 = time.perf_counter()
+    for i in range(steps):
+        table.init(stream=g.stream)
+        evs[i][0].record(g.stream)
+        table.insert(keys, stream=g.stream)
+        evs[i][1].record(g.stream)
+    g.stream.synchronize()
+    wall = time.perf_counter() - t0
+    t = [e[0].elapsed_time(e[1]) for e in evs]
+    distinct = table.size(stream=g.stream)
+    res = dict(n=n, distinct=distinct, capacity=table.capacity(), wall_ms_per_step=1e3 * wall / steps,
+               insert_ms=sum(t) / steps, insert_ms_min=min(t))
+    del table, keys
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_baseline(log2n: int = 24, reps: int = 3):
+    """OpenMP table with the same hashing, probing and CAS protocol, on a
+    2^log2n-key sample of the same workload (u64/u64, LF 0.9, 50 % hits)."""
+    import numpy as np
+    import oracle
+    lib = oracle.lib()
+    n = 1 << log2n
+    idx = np.arange(0x1000, 0x1000 + n + n // 2, dtype=np.uint64)
+    allk = np.empty_like(idx)
+    lib.wco_fmix64_array(oracle._p(idx), oracle._p(allk), len(idx))
+    allk = allk[(allk != 0) & (allk != np.uint64(2**64 - 1))]
+    keys, absent = allk[:n], allk[n:n + n // 2]
+    vals = np.random.default_rng(0xC2).integers(0, 2**63, n, dtype=np.uint64)
+    queries = np.empty(n, np.uint64)
+    queries[0::2], queries[1::2] = keys[: n // 2], absent
+    out = np.zeros(n, np.uint64)
+    t = oracle.OmpTable(math.ceil(n / 0.9), 64, 64, cg=8)
+    best = float("inf")
+    for _ in range(reps):
+        t.init()
+        ti, _ = t.insert(keys, vals, status=False)
+        tr, _ = t.retrieve(queries, out, status=False)
+        best = min(best, ti + tr)
+    return dict(value=round(2 * n / best / 1e9, 4), unit="Gops/s", cores=oracle.omp_threads(), kind="port",
+                sample=f"u64/u64 OpenMP table (same fmix64 double hashing, cg-8 windows, std::atomic CAS), "
+                       f"2^{log2n} unique keys at LF 0.9 then 2^{log2n} lookups at 50% hits; "
+                       f"best of {reps} insert+retrieve passes")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log2-keys", type=int, default=28)
+    ap.add_argument("--no-extras", action="store_true", help="skip the u32 and counting workloads")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-log2-keys", type=int, default=24)
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (PMC calibration)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on exactly one GPU: the hash-table hot path has no multi-GPU "
+              "path in this build (sharding is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    g = Gpu()
+    log(f"[bench] svht u64/u64 2^{args.log2_keys} keys, steps={args.steps} warmup={args.warmup}")
+    r64 = run_svht(g, 64, 64, 8, args.log2_keys, args.steps, args.warmup)
+    log(f"[bench] u64: {r64}")
+    extras = {}
+    if not args.no_extras:
+        r32 = run_svht(g, 32, 32, 16, args.log2_keys, args.steps, args.warmup)
+        log(f"[bench] u32: {r32}")
+        rc = run_counting(g, args.log2_keys, 24, 1.1, args.steps, args.warmup)
+        log(f"[bench] counting: {rc}")
+        extras = {
+            "u32u32_cg16": {"gops": round(2 * r32["n"] / r32["wall_ms_per_step"] / 1e6, 3),
+                            "insert_gops": round(r32["n"] / r32["insert_ms"] / 1e6, 3),
+                            "retrieve_gops": round(r32["n"] / r32["retrieve_ms"] / 1e6, 3),
+                            "capacity": r32["capacity"]},
+            "counting_u64_zipf1.1": {"gops": round(rc["n"] / rc["wall_ms_per_step"] / 1e6, 3),
+                                     "insert_gops": round(rc["n"] / rc["insert_ms"] / 1e6, 3),
+                                     "occurrences": rc["n"], "distinct": rc["distinct"],
+                                     "capacity": rc["capacity"]},
+        }
+    if args.calibrate_copy_mib:
+        torch = g.torch
+        a = torch.empty(args.calibrate_copy_mib << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.calibrate_copy_mib} MiB done")
+
+    n = r64["n"]
+    per_op = {"insert": algorithmic_bytes("insert", 64, 64), "retrieve": algorithmic_bytes("retrieve", 64, 64)}
+    kt = {"insert": r64["insert_ms"], "retrieve": r64["retrieve_ms"]}
+    dom = max(kt, key=kt.get)
+    achieved = per_op[dom] * n / (kt[dom] * 1e-3) / 1e9
+    other = "retrieve" if dom == "insert" else "insert"
+    achieved_other = per_op[other] * n / (kt[other] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": f"svht_u64u64_{dom}", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(dom),
+                "bytes_per_op": per_op[dom], "ops_per_launch": n, "avg_kernel_ms": round(kt[dom], 4),
+                "other_kernel": {"kernel": f"svht_u64u64_{other}", "achieved": round(achieved_other, 1),
+                                 "frac": round(achieved_other / HBM_PEAK_GBS, 4), "bytes_per_op": per_op[other],
+                                 "avg_kernel_ms": round(kt[other], 4), "traffic": read_traffic(other)}}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(args.cpu_log2_keys)
+        log(f"[bench] cpu: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(2 * n / r64["wall_ms_per_step"] / 1e6, 3),
+        "unit": "Gops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r64["wall_ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic: 2^28 distinct fmix64-scrambled u64 keys, splitmix64 u64 values, "
+                "2^28 lookups (2^27 stored + 2^27 absent keys)",
+        "config": {"workload": f"SingleValueHashTable<u64,u64> init+insert+retrieve, 2^{args.log2_keys} keys, "
+                               f"LF 0.9 ({r64['capacity']} slots, {r64['table_bytes'] / 1e9:.2f} GB), cg 8, "
+                               f"50% hit lookups",
+                   "keys": n, "capacity": r64["capacity"], "cg_size": 8,
+                   "insert_gops": round(n / r64["insert_ms"] / 1e6, 3),
+                   "retrieve_gops": round(n / r64["retrieve_ms"] / 1e6, 3),
+                   "kernel_only_gops": round(2 * n / (r64["insert_ms"] + r64["retrieve_ms"]) / 1e6, 3)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    out.update(extras)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -147,4 +147,126 @@ WC_API int wc_svht_u64u32_is_initialized(void* table);
 WC_API uint32_t wc_svht_u64u32_cg_size(void* table);
 
 
+/* ------------------------------------------------------------------------
+ * CountingHashTable<K, uint64_t> (include/warpcore/counting_hash_table.cuh)
+ *   _create (min_capacity, seed, max_count, no_init)               :74-85
+ *   _init / _init_seed                                              :113-135
+ *   _insert   insert(keys, n, stream, probing_length, status_out)  :182-194
+ *   _retrieve counts_out[i] = min(count, max_count) on a hit       :236-249
+ *   _erase    erase(keys, n, stream, probing_length, status_out)   :273-301
+ *   _retrieve_all / _size / _load_factor / _capacity / _peek_status /
+ *   _pop_status                                                    :262-390
+ * cg_size 1..64, default 4.
+ * ---------------------------------------------------------------------- */
+/* CountingHashTable<uint32_t, uint64_t> */
+WC_API void* wc_counting_u32_create(uint64_t min_capacity, uint32_t seed, uint64_t max_count, int no_init, uint32_t cg_size);
+WC_API void wc_counting_u32_destroy(void* table);
+WC_API int wc_counting_u32_init(void* table, void* stream);
+WC_API int wc_counting_u32_init_seed(void* table, uint32_t seed, void* stream);
+WC_API int wc_counting_u32_insert(void* table, const uint32_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_counting_u32_retrieve(void* table, const uint32_t* keys_in, uint64_t num_in, uint64_t* counts_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_counting_u32_erase(void* table, const uint32_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_counting_u32_retrieve_all(void* table, uint32_t* keys_out, uint64_t* counts_out, uint64_t* num_out, void* stream);
+WC_API uint64_t wc_counting_u32_size(void* table, void* stream);
+WC_API float wc_counting_u32_load_factor(void* table, void* stream);
+WC_API uint64_t wc_counting_u32_capacity(void* table);
+WC_API uint64_t wc_counting_u32_bytes_total(void* table);
+WC_API uint32_t wc_counting_u32_peek_status(void* table, void* stream);
+WC_API uint32_t wc_counting_u32_pop_status(void* table, void* stream);
+WC_API uint32_t wc_counting_u32_cg_size(void* table);
+
+/* CountingHashTable<uint64_t, uint64_t> */
+WC_API void* wc_counting_u64_create(uint64_t min_capacity, uint64_t seed, uint64_t max_count, int no_init, uint32_t cg_size);
+WC_API void wc_counting_u64_destroy(void* table);
+WC_API int wc_counting_u64_init(void* table, void* stream);
+WC_API int wc_counting_u64_init_seed(void* table, uint64_t seed, void* stream);
+WC_API int wc_counting_u64_insert(void* table, const uint64_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_counting_u64_retrieve(void* table, const uint64_t* keys_in, uint64_t num_in, uint64_t* counts_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_counting_u64_erase(void* table, const uint64_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_counting_u64_retrieve_all(void* table, uint64_t* keys_out, uint64_t* counts_out, uint64_t* num_out, void* stream);
+WC_API uint64_t wc_counting_u64_size(void* table, void* stream);
+WC_API float wc_counting_u64_load_factor(void* table, void* stream);
+WC_API uint64_t wc_counting_u64_capacity(void* table);
+WC_API uint64_t wc_counting_u64_bytes_total(void* table);
+WC_API uint32_t wc_counting_u64_peek_status(void* table, void* stream);
+WC_API uint32_t wc_counting_u64_pop_status(void* table, void* stream);
+WC_API uint32_t wc_counting_u64_cg_size(void* table);
+
+/* ------------------------------------------------------------------------
+ * HashSet<K> (include/warpcore/hash_set.cuh)
+ *   _create (min_capacity, seed)                                   :80-110
+ *   _insert                                                        :264-282
+ *   _contains flags_out[i] = membership, written when the status is none;
+ *             the device retrieve semantics of :291-329 (the host retrieve
+ *             of :340-359 cannot be instantiated)
+ *   _erase (status none whatever it finds)                         :446-462
+ *   _retrieve_all / _size / _load_factor / _capacity / _peek_status /
+ *   _pop_status                                                    :366-640
+ * cg_size 1..64, default 16.
+ * ---------------------------------------------------------------------- */
+/* HashSet<uint32_t> */
+WC_API void* wc_hashset_u32_create(uint64_t min_capacity, uint32_t seed, uint32_t cg_size);
+WC_API void wc_hashset_u32_destroy(void* set);
+WC_API int wc_hashset_u32_init(void* set, void* stream);
+WC_API int wc_hashset_u32_init_seed(void* set, uint32_t seed, void* stream);
+WC_API int wc_hashset_u32_insert(void* set, const uint32_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_hashset_u32_contains(void* set, const uint32_t* keys_in, uint64_t num_in, bool* flags_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_hashset_u32_erase(void* set, const uint32_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_hashset_u32_retrieve_all(void* set, uint32_t* keys_out, uint64_t* num_out, void* stream);
+WC_API uint64_t wc_hashset_u32_size(void* set, void* stream);
+WC_API float wc_hashset_u32_load_factor(void* set, void* stream);
+WC_API uint64_t wc_hashset_u32_capacity(void* set);
+WC_API uint64_t wc_hashset_u32_bytes_total(void* set);
+WC_API uint32_t wc_hashset_u32_peek_status(void* set, void* stream);
+WC_API uint32_t wc_hashset_u32_pop_status(void* set, void* stream);
+WC_API uint32_t wc_hashset_u32_cg_size(void* set);
+
+/* HashSet<uint64_t> */
+WC_API void* wc_hashset_u64_create(uint64_t min_capacity, uint64_t seed, uint32_t cg_size);
+WC_API void wc_hashset_u64_destroy(void* set);
+WC_API int wc_hashset_u64_init(void* set, void* stream);
+WC_API int wc_hashset_u64_init_seed(void* set, uint64_t seed, void* stream);
+WC_API int wc_hashset_u64_insert(void* set, const uint64_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_hashset_u64_contains(void* set, const uint64_t* keys_in, uint64_t num_in, bool* flags_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_hashset_u64_erase(void* set, const uint64_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_hashset_u64_retrieve_all(void* set, uint64_t* keys_out, uint64_t* num_out, void* stream);
+WC_API uint64_t wc_hashset_u64_size(void* set, void* stream);
+WC_API float wc_hashset_u64_load_factor(void* set, void* stream);
+WC_API uint64_t wc_hashset_u64_capacity(void* set);
+WC_API uint64_t wc_hashset_u64_bytes_total(void* set);
+WC_API uint32_t wc_hashset_u64_peek_status(void* set, void* stream);
+WC_API uint32_t wc_hashset_u64_pop_status(void* set, void* stream);
+WC_API uint32_t wc_hashset_u64_cg_size(void* set);
+
+/* ------------------------------------------------------------------------
+ * Synthetic inputs (the role of include/warpcore/random_distributions.cuh:
+ * uniform 50-135, unique 144-246, zipf 259-326). Counter-based: element i
+ * depends only on (seed or offset, i).
+ *   _unique_*   out[i] = fmix(offset + i), never 0 or ~0, all distinct
+ *   _uniform_*  out[i] = splitmix64(seed, i)
+ *   _zipf_*     Zipf(s) ranks over num_distinct keys (rejection-inversion,
+ *               any s > 0), rank k -> unique key (key_offset, k - 1)
+ *   _interleave_* out[2j], out[2j+1] = a[j], b[j] in hashed order
+ * ---------------------------------------------------------------------- */
+WC_API int wc_gen_unique_u32(uint32_t* out, uint64_t n, uint64_t offset, void* stream);
+WC_API int wc_gen_unique_u64(uint64_t* out, uint64_t n, uint64_t offset, void* stream);
+WC_API int wc_gen_uniform_u32(uint32_t* out, uint64_t n, uint64_t seed, void* stream);
+WC_API int wc_gen_uniform_u64(uint64_t* out, uint64_t n, uint64_t seed, void* stream);
+WC_API int wc_gen_zipf_u32(uint32_t* out, uint64_t n, uint64_t num_distinct, double s, uint64_t seed, uint64_t key_offset, void* stream);
+WC_API int wc_gen_zipf_u64(uint64_t* out, uint64_t n, uint64_t num_distinct, double s, uint64_t seed, uint64_t key_offset, void* stream);
+WC_API int wc_gen_interleave_u32(uint32_t* out, const uint32_t* a, const uint32_t* b, uint64_t half, uint64_t seed, void* stream);
+WC_API int wc_gen_interleave_u64(uint64_t* out, const uint64_t* a, const uint64_t* b, uint64_t half, uint64_t seed, void* stream);
+
+/* Capacity rule detail::get_valid_capacity (include/warpcore/base.cuh:27-34)
+ * and the unique-key map, evaluated on the host. */
+WC_API uint64_t wc_get_valid_capacity(uint64_t min_capacity, uint64_t cg_size);
+WC_API uint64_t wc_unique_key_u64(uint64_t offset, uint64_t i);
+WC_API uint32_t wc_unique_key_u32(uint64_t offset, uint64_t i);
+
+/* Kernel-shape policy (no reference counterpart): bulk operations on tables
+ * whose slots take at most `bytes` are probed one lane per key, larger
+ * tables by cooperative groups of cg_size lanes per key. */
+WC_API void wc_set_lane_table_bytes(uint64_t bytes);
+WC_API uint64_t wc_get_lane_table_bytes(void);
+
 #endif /* WARPCORE_C_H */

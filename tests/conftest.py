@@ -29,3 +29,11 @@ def oracle():
     import oracle as o
     o.lib()
     return o
+
+
+@pytest.fixture(params=["lanes", "groups"])
+def shape(request):
+    """Runs a GPU test once per bulk-kernel shape."""
+    from datagen import kernel_shape
+    with kernel_shape(request.param):
+        yield request.param

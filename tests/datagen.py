@@ -75,3 +75,23 @@ def dev_empty(n: int, bits: int, fill: int = 0):
         a = np.full(max(n, 1), fill, dtype=dtype(bits) if bits in (32, 64) else np.uint8)
         t.copy_(to_dev(a))
     return t[:n] if n > 0 else t[:0]
+
+
+class kernel_shape:
+    """Context manager selecting the bulk-kernel shape: 'lanes' (one lane per
+    key) or 'groups' (cooperative groups), by moving the table-size limit of
+    the lane-per-key path."""
+
+    def __init__(self, shape: str):
+        self.limit = (1 << 62) if shape == "lanes" else 0
+
+    def __enter__(self):
+        from warpcore_amd import _capi
+        self.lib = _capi.lib()
+        self.saved = self.lib.wc_get_lane_table_bytes()
+        self.lib.wc_set_lane_table_bytes(self.limit)
+        return self
+
+    def __exit__(self, *exc):
+        self.lib.wc_set_lane_table_bytes(self.saved)
+        return False

@@ -53,7 +53,7 @@ def _oracle_retrieve(o, keys, vb, pl=None):
     (64, 64, 8, 1 << 18), (64, 64, 4, 1 << 16), (64, 64, 16, 1 << 16), (64, 64, 64, 1 << 16),
     (32, 64, 8, 1 << 16), (64, 32, 8, 1 << 16),
 ])
-def test_insert_retrieve_parity(cuda, oracle, kb, vb, cg, n):
+def test_insert_retrieve_parity(cuda, shape, oracle, kb, vb, cg, n):
     print(f"[svht parity] kb={kb} vb={vb} cg={cg} n={n}", flush=True)
     keys = unique_keys(n, kb, seed=0xC1)
     vals = uniform_values(n, vb, seed=0xC2)
@@ -83,7 +83,7 @@ def test_insert_retrieve_parity(cuda, oracle, kb, vb, cg, n):
 
 
 @pytest.mark.parametrize("kb,vb,cg", [(32, 32, 8), (64, 64, 8), (32, 32, 16)])
-def test_duplicates_one_none_per_key(cuda, oracle, kb, vb, cg):
+def test_duplicates_one_none_per_key(cuda, shape, oracle, kb, vb, cg):
     n_keys = 1 << 14
     keys = unique_keys(n_keys, kb, seed=11)
     rng = np.random.default_rng(12)
@@ -113,7 +113,7 @@ def test_duplicates_one_none_per_key(cuda, oracle, kb, vb, cg):
 
 
 @pytest.mark.parametrize("kb,vb,cg", [(32, 32, 8), (64, 64, 8)])
-def test_erase_and_reinsert(cuda, oracle, kb, vb, cg):
+def test_erase_and_reinsert(cuda, shape, oracle, kb, vb, cg):
     n = 1 << 16
     keys = unique_keys(n, kb, seed=21)
     vals = uniform_values(n, vb, seed=22)
@@ -145,7 +145,7 @@ def test_erase_and_reinsert(cuda, oracle, kb, vb, cg):
 
 
 @pytest.mark.parametrize("kb", [32, 64])
-def test_invalid_keys(cuda, oracle, kb):
+def test_invalid_keys(cuda, shape, oracle, kb):
     keys = np.array([0, tomb(kb), 5, 0, 7], dtype(kb))
     vals = np.arange(5, dtype=dtype(kb))
     t, o = _tables(oracle, 1000, kb, kb, 8)
@@ -164,7 +164,7 @@ def test_invalid_keys(cuda, oracle, kb):
 
 
 @pytest.mark.parametrize("kb,vb,cg", [(32, 32, 1), (64, 64, 2)])
-def test_full_table_probing_length_exceeded(cuda, oracle, kb, vb, cg):
+def test_full_table_probing_length_exceeded(cuda, shape, oracle, kb, vb, cg):
     """A table with no EMPTY slot: every new key and every miss exhausts probing.
 
     The reference would probe forever here (default probing length ~2^64);
@@ -220,7 +220,7 @@ def test_no_init_is_inert(cuda):
 
 
 @pytest.mark.parametrize("kb,vb", [(32, 32), (64, 64)])
-def test_retrieve_all_and_size(cuda, oracle, kb, vb):
+def test_retrieve_all_and_size(cuda, shape, oracle, kb, vb):
     n = 50000
     keys = unique_keys(n, kb, seed=41)
     vals = uniform_values(n, vb, seed=42)
@@ -237,7 +237,7 @@ def test_retrieve_all_and_size(cuda, oracle, kb, vb):
     np.testing.assert_array_equal(v[order], vals[ref])
 
 
-def test_bool_handler(cuda, oracle):
+def test_bool_handler(cuda, shape, oracle):
     keys = np.array([3, 0, 9, 3], np.uint32)
     vals = np.array([1, 2, 3, 4], np.uint32)
     t, _ = _tables(oracle, 1000, 32, 32, 8)

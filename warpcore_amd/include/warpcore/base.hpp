@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstdint>
 #include <cstdio>
@@ -59,6 +60,57 @@ WC_H unsigned grid_for(index_t groups, index_t cg, index_t waves_per_cu = 32) no
     const index_t needed = SDIV(threads, block_size());
     const index_t cap = index_t(num_compute_units()) * waves_per_cu * wave_size() / block_size();
     index_t g = std::min(needed, cap);
+    return unsigned(g == 0 ? 1 : g);
+}
+
+// Keys each cooperative group keeps in flight in the batched probing kernels.
+#ifndef WARPCORE_KEYS_PER_GROUP
+#define WARPCORE_KEYS_PER_GROUP 4
+#endif
+constexpr int keys_per_group() noexcept { return WARPCORE_KEYS_PER_GROUP; }
+
+// Grid of a batched probing launch: one wavefront takes 64 / cg groups of
+// keys_per_group() keys per round; at most 8 blocks of 256 per CU.
+WC_H unsigned grid_for_batched(index_t items, index_t cg) noexcept {
+    const index_t per_wave = (wave_size() / cg) * index_t(keys_per_group());
+    const index_t waves = SDIV(items, per_wave);
+    const index_t blocks = SDIV(waves, block_size() / wave_size());
+    const index_t cap = index_t(num_compute_units()) * 8;
+    const index_t g = std::min(blocks, cap);
+    return unsigned(g == 0 ? 1 : g);
+}
+
+// Keys each lane keeps in flight in the lane-per-key probing kernels.
+#ifndef WARPCORE_KEYS_PER_LANE
+#define WARPCORE_KEYS_PER_LANE 2
+#endif
+constexpr int keys_per_lane() noexcept { return WARPCORE_KEYS_PER_LANE; }
+
+// Largest table (bytes of slots) probed lane-per-key. One lane reading a
+// whole 128-byte window issues 8 loads, each translating 64 addresses, so
+// beyond a few GB the address translation caches miss and 8-lane
+// cooperative windows (one translation per line) are faster.
+// The limit is a process-wide setting (set_lane_table_bytes) so a tuner or
+// a test can move it.
+#ifndef WARPCORE_LANE_TABLE_BYTES
+#define WARPCORE_LANE_TABLE_BYTES (std::uint64_t(1) << 31)
+#endif
+inline std::atomic<std::uint64_t>& lane_table_bytes_setting() noexcept {
+    static std::atomic<std::uint64_t> bytes{WARPCORE_LANE_TABLE_BYTES};
+    return bytes;
+}
+WC_H std::uint64_t lane_table_bytes() noexcept { return lane_table_bytes_setting().load(std::memory_order_relaxed); }
+WC_H void set_lane_table_bytes(std::uint64_t bytes) noexcept {
+    lane_table_bytes_setting().store(bytes, std::memory_order_relaxed);
+}
+
+// Grid of a lane-per-key launch: keys_per_lane() keys per lane, at most 8
+// blocks of 256 per CU.
+WC_H unsigned grid_for_lanes(index_t items) noexcept {
+    const index_t lanes = SDIV(items, index_t(keys_per_lane()));
+    const index_t blocks = SDIV(lanes, block_size());
+    const index_t cap = index_t(num_compute_units()) * 8;
+    const index_t g = std::min(blocks, cap);
     return unsigned(g == 0 ? 1 : g);
 }
 

@@ -17,6 +17,8 @@
 #include "base.hpp"
 #include "capacity.hpp"
 #include "defaults.hpp"
+#include "batched_probing.hpp"
+#include "lane_probing.hpp"
 #include "engine.hpp"
 #include "probing_schemes.hpp"
 #include "status.hpp"
@@ -42,7 +44,11 @@ struct SingleValueView {
     using value_type = Value;
     using slot_type = Slot;
     using index_type = index_t;
+    using probing_scheme_type = ProbingScheme;
+    using out_type = Value;
+    using probe_slot_type = Slot;
     static constexpr bool packed = is_packed_pair<Slot>();
+    static constexpr bool publishes_value = packed;
 
     Slot* slots;
     Status* status;
@@ -52,12 +58,78 @@ struct SingleValueView {
 
     WC_HD static constexpr index_type cg_size() noexcept { return ProbingScheme::cg_size(); }
     WC_HD static constexpr bool is_empty_key(Key k) noexcept { return k == EmptyKey; }
+    WC_HD static constexpr Key empty_key_value() noexcept { return EmptyKey; }
     WC_HD static constexpr bool is_tombstone_key(Key k) noexcept { return k == TombstoneKey; }
     WC_HD static constexpr bool is_valid_key(Key k) noexcept { return k != EmptyKey && k != TombstoneKey; }
     WC_HD index_type capacity() const noexcept { return geo.capacity; }
     WC_HD Status* status_word() const noexcept { return status; }
     WC_D Key key_at(index_type i) const noexcept { return slots[i].key; }
     WC_D Slot slot_at(index_type i) const noexcept { return load_slot(slots + i); }
+
+    // ---- hooks of the batched kernels (batched_probing.hpp)
+    WC_HD bool ready() const noexcept { return initialized; }
+    WC_HD Key hash_seed() const noexcept { return seed; }
+    WC_HD const probing_schemes::Geometry& geometry() const noexcept { return geo; }
+    WC_HD Slot* slot_ptr() const noexcept { return slots; }
+    WC_HD static Key key_of(const Slot& s) noexcept { return s.key; }
+    WC_HD static Value value_of(const Slot& s) noexcept { return s.value; }
+    WC_D Value found(const Value& v) const noexcept { return v; }
+    WC_D Status missing(Value&) const noexcept { return Status::key_not_found(); }
+    // Claiming CAS of the batched insert, split so that the U CASes of a
+    // group are all in flight before any result is examined. cas_issue()
+    // swaps the EMPTY slot this lane saw as `seen` for the key (packed slots:
+    // key and value in one 64-bit word) and returns the old word;
+    // cas_outcome() classifies it: 1 claimed, 2 the key was already there,
+    // 0 another key won the slot, 3 the EMPTY slot's value bits moved (retry
+    // with `seen` refreshed).
+    using cas_word = typename std::conditional<packed, std::uint64_t, Key>::type;
+    // what insert reads of a slot: the whole 8-byte packed pair, else the key
+    using insert_probe_type = typename std::conditional<packed, Slot, Key>::type;
+    WC_D static insert_probe_type load_insert_probe(const Slot* p) noexcept {
+        if constexpr (packed) return load_slot(p);
+        else return p->key;
+    }
+    // expected word of a claiming CAS on an EMPTY slot: EMPTY and the value
+    // bits the EMPTY fill writes (cas_outcome() refreshes it if they differ)
+    WC_HD static insert_probe_type empty_probe() noexcept {
+        if constexpr (packed) {
+            Slot s;
+            s.key = EmptyKey;
+            s.value = Value{};
+            return s;
+        } else {
+            return EmptyKey;
+        }
+    }
+    WC_HD static Key probe_key(const insert_probe_type& s) noexcept {
+        if constexpr (packed) return s.key;
+        else return s;
+    }
+    WC_D cas_word cas_issue(Slot* p, const insert_probe_type& seen, Key key_in, const Value& value_in) const noexcept {
+        if constexpr (packed) {
+            Slot want;
+            want.key = key_in;
+            want.value = value_in;
+            return atomic_cas(reinterpret_cast<std::uint64_t*>(p), slot_bits(seen), slot_bits(want));
+        } else {
+            return atomic_cas(&(p->key), seen, key_in);
+        }
+    }
+    WC_D int cas_outcome(cas_word old, insert_probe_type& seen, Key key_in) const noexcept {
+        if constexpr (packed) {
+            if (old == slot_bits(seen)) return 1;
+            const Key old_key = slot_from_bits<Slot>(old).key;
+            if (old_key == EmptyKey) { seen = slot_from_bits<Slot>(old); return 3; }
+            return old_key == key_in ? 2 : 0;
+        } else {
+            return old == seen ? 1 : (old == key_in ? 2 : 0);
+        }
+    }
+    // after insert_batched found the key's slot (single_value_hash_table.cuh:222-225)
+    template<class Join>
+    WC_D void finish_insert(index_type slot, const Value& value_in, bool published, Status&, Join&) const noexcept {
+        if (!published) slots[slot].value = value_in;
+    }
 
     // Finds or claims the slot of `key_in`. Returns the slot index, or ~0 if
     // none (invalid key, probing exhausted). With `publish` and a packed
@@ -292,9 +364,17 @@ public:
                      typename StatusHandler::base_type* status_out = nullptr) noexcept {
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!is_initialized_ || num_in == 0) return;
-        kernels::insert<view_type, StatusHandler>
-            <<<detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(keys_in, values_in, num_in, view(),
-                                                                                probing_length, status_out);
+        if constexpr (detail::LaneWindow<slot_type, cg_size()>::usable) {
+            if (capacity() * sizeof(slot_type) <= detail::lane_table_bytes()) {
+                kernels::insert_lanes<view_type, StatusHandler, detail::keys_per_lane(), true>
+                    <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, view(),
+                                                                                 probing_length, status_out);
+                return;
+            }
+        }
+        kernels::insert_batched<view_type, StatusHandler, detail::keys_per_group(), true>
+            <<<detail::grid_for_batched(num_in, cg_size()), block_size(), 0, stream>>>(
+                keys_in, values_in, num_in, view(), probing_length, status_out);
     }
 
     // single_value_hash_table.cuh:318-337
@@ -304,9 +384,17 @@ public:
                        typename StatusHandler::base_type* status_out = nullptr) const noexcept {
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!is_initialized_ || num_in == 0) return;
-        kernels::retrieve<view_type, StatusHandler>
-            <<<detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(keys_in, num_in, values_out, view(),
-                                                                                probing_length, status_out);
+        if constexpr (detail::LaneWindow<slot_type, cg_size()>::usable) {
+            if (capacity() * sizeof(slot_type) <= detail::lane_table_bytes()) {
+                kernels::retrieve_lanes<view_type, StatusHandler, detail::keys_per_lane()>
+                    <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, num_in, values_out, view(),
+                                                                                 probing_length, status_out);
+                return;
+            }
+        }
+        kernels::retrieve_batched<view_type, StatusHandler, detail::keys_per_group()>
+            <<<detail::grid_for_batched(num_in, cg_size()), block_size(), 0, stream>>>(
+                keys_in, num_in, values_out, view(), probing_length, status_out);
     }
 
     // single_value_hash_table.cuh:390-417 (order unspecified)

@@ -13,3 +13,6 @@
 #include "defaults.hpp"
 #include "engine.hpp"
 #include "single_value_hash_table.hpp"
+#include "counting_hash_table.hpp"
+#include "hash_set.hpp"
+#include "random_distributions.hpp"

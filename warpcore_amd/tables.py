@@ -158,3 +158,141 @@ class SingleValueHashTable:
 
     def cg_size(self) -> int:
         return self._fn("cg_size")(self._h)
+
+
+class CountingHashTable:
+    """CountingHashTable<K, uint64_t> (include/warpcore/counting_hash_table.cuh)."""
+
+    def __init__(self, min_capacity: int, key_bits: int = 64, seed: int = DEFAULT_SEED,
+                 max_count: int = (1 << 64) - 1, no_init: bool = False, cg_size: int = 0):
+        self.key_bits = key_bits
+        self._p = f"wc_counting_u{key_bits}_"
+        self._lib = _capi.lib()
+        self._h = self._fn("create")(min_capacity, seed, max_count, int(no_init), cg_size)
+        if not self._h:
+            raise ValueError(f"unsupported configuration (cg_size={cg_size})")
+
+    def _fn(self, name):
+        return getattr(self._lib, self._p + name)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._fn("destroy")(h)
+
+    def init(self, seed: int | None = None, stream=None):
+        if seed is None:
+            _check(self._fn("init")(self._h, _stream(stream)), "init")
+        else:
+            _check(self._fn("init_seed")(self._h, seed, _stream(stream)), "init")
+
+    def insert(self, keys_in, stream=None, probing_length=DEFAULT_PROBING_LENGTH, status_out=None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _check(self._fn("insert")(self._h, _ptr(keys_in), keys_in.numel(), _stream(stream), probing_length,
+                                  _ptr(status_out)), "insert")
+
+    def retrieve(self, keys_in, counts_out, stream=None, probing_length=DEFAULT_PROBING_LENGTH, status_out=None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(counts_out, 64, "counts_out")
+        _check(self._fn("retrieve")(self._h, _ptr(keys_in), keys_in.numel(), _ptr(counts_out), _stream(stream),
+                                    probing_length, _ptr(status_out)), "retrieve")
+
+    def erase(self, keys_in, stream=None, probing_length=DEFAULT_PROBING_LENGTH, status_out=None):
+        _check(self._fn("erase")(self._h, _ptr(keys_in), keys_in.numel(), _stream(stream), probing_length,
+                                 _ptr(status_out)), "erase")
+
+    def retrieve_all(self, keys_out, counts_out, stream=None) -> int:
+        n = ctypes.c_uint64(0)
+        _check(self._fn("retrieve_all")(self._h, _ptr(keys_out), _ptr(counts_out), ctypes.byref(n),
+                                        _stream(stream)), "retrieve_all")
+        return n.value
+
+    def size(self, stream=None) -> int:
+        return self._fn("size")(self._h, _stream(stream))
+
+    def load_factor(self, stream=None) -> float:
+        return self._fn("load_factor")(self._h, _stream(stream))
+
+    def capacity(self) -> int:
+        return self._fn("capacity")(self._h)
+
+    def bytes_total(self) -> int:
+        return self._fn("bytes_total")(self._h)
+
+    def peek_status(self, stream=None) -> Status:
+        return Status(self._fn("peek_status")(self._h, _stream(stream)))
+
+    def pop_status(self, stream=None) -> Status:
+        return Status(self._fn("pop_status")(self._h, _stream(stream)))
+
+    def cg_size(self) -> int:
+        return self._fn("cg_size")(self._h)
+
+
+class HashSet:
+    """HashSet<K> (include/warpcore/hash_set.cuh)."""
+
+    def __init__(self, min_capacity: int, key_bits: int = 32, seed: int = DEFAULT_SEED, cg_size: int = 0):
+        self.key_bits = key_bits
+        self._p = f"wc_hashset_u{key_bits}_"
+        self._lib = _capi.lib()
+        self._h = self._fn("create")(min_capacity, seed, cg_size)
+        if not self._h:
+            raise ValueError(f"unsupported configuration (cg_size={cg_size})")
+
+    def _fn(self, name):
+        return getattr(self._lib, self._p + name)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._fn("destroy")(h)
+
+    def init(self, seed: int | None = None, stream=None):
+        if seed is None:
+            _check(self._fn("init")(self._h, _stream(stream)), "init")
+        else:
+            _check(self._fn("init_seed")(self._h, seed, _stream(stream)), "init")
+
+    def insert(self, keys_in, stream=None, probing_length=DEFAULT_PROBING_LENGTH, status_out=None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _check(self._fn("insert")(self._h, _ptr(keys_in), keys_in.numel(), _stream(stream), probing_length,
+                                  _ptr(status_out)), "insert")
+
+    def contains(self, keys_in, flags_out, stream=None, probing_length=DEFAULT_PROBING_LENGTH, status_out=None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(flags_out, 8, "flags_out")
+        _check(self._fn("contains")(self._h, _ptr(keys_in), keys_in.numel(), _ptr(flags_out), _stream(stream),
+                                    probing_length, _ptr(status_out)), "contains")
+
+    retrieve = contains
+
+    def erase(self, keys_in, stream=None, probing_length=DEFAULT_PROBING_LENGTH, status_out=None):
+        _check(self._fn("erase")(self._h, _ptr(keys_in), keys_in.numel(), _stream(stream), probing_length,
+                                 _ptr(status_out)), "erase")
+
+    def retrieve_all(self, keys_out, stream=None) -> int:
+        n = ctypes.c_uint64(0)
+        _check(self._fn("retrieve_all")(self._h, _ptr(keys_out), ctypes.byref(n), _stream(stream)), "retrieve_all")
+        return n.value
+
+    def size(self, stream=None) -> int:
+        return self._fn("size")(self._h, _stream(stream))
+
+    def load_factor(self, stream=None) -> float:
+        return self._fn("load_factor")(self._h, _stream(stream))
+
+    def capacity(self) -> int:
+        return self._fn("capacity")(self._h)
+
+    def bytes_total(self) -> int:
+        return self._fn("bytes_total")(self._h)
+
+    def peek_status(self, stream=None) -> Status:
+        return Status(self._fn("peek_status")(self._h, _stream(stream)))
+
+    def pop_status(self, stream=None) -> Status:
+        return Status(self._fn("pop_status")(self._h, _stream(stream)))
+
+    def cg_size(self) -> int:
+        return self._fn("cg_size")(self._h)
