@@ -239,6 +239,63 @@ WC_API uint32_t wc_hashset_u64_pop_status(void* set, void* stream);
 WC_API uint32_t wc_hashset_u64_cg_size(void* set);
 
 /* ------------------------------------------------------------------------
+ * BloomFilter<K, MurmurHash<K>, Slot, CG> (include/warpcore/bloom_filter.cuh)
+ *   create       constructor :77-93; slot_bits 32 or 64 (0 = 64), cg_size
+ *                1..64 power of two (0 = 1); NULL for num_bits == 0, k == 0
+ *                or an unsupported configuration
+ *   init         :142-161 (clears all bits; _init_seed also re-seeds)
+ *   insert       bulk insert :199-208 (device member :167-192)
+ *   retrieve     bulk query :245-255 (device member :215-237); flags_out[i]
+ *                is 1 iff every bit of keys_in[i] is set
+ *   insert_and_query  bulk form of the device member :263-297, cg_size 1
+ *                only (else -2); flag = all bits were set before the key's
+ *                own update
+ *   copy_words   copies the num_slots words of the bit array to a device
+ *                buffer of bytes_total bytes (no reference counterpart; it
+ *                exposes the array the reference keeps private)
+ *   fpr          blocked-model false-positive estimate :341-408, summed in
+ *                log space so it stays finite for large n
+ *   num_bits/num_slots/num_blocks/k :304-338
+ * ---------------------------------------------------------------------- */
+/* BloomFilter<uint32_t> */
+WC_API void* wc_bloom_u32_create(uint64_t num_bits, uint64_t k, uint32_t seed, uint32_t slot_bits, uint32_t cg_size);
+WC_API void wc_bloom_u32_destroy(void* filter);
+WC_API int wc_bloom_u32_init(void* filter, void* stream);
+WC_API int wc_bloom_u32_init_seed(void* filter, uint32_t seed, void* stream);
+WC_API int wc_bloom_u32_insert(void* filter, const uint32_t* keys_in, uint64_t num_in, void* stream);
+WC_API int wc_bloom_u32_retrieve(void* filter, const uint32_t* keys_in, uint64_t num_in, bool* flags_out, void* stream);
+WC_API int wc_bloom_u32_insert_and_query(void* filter, const uint32_t* keys_in, uint64_t num_in, bool* flags_out, void* stream);
+WC_API int wc_bloom_u32_copy_words(void* filter, void* words_out, void* stream);
+WC_API uint64_t wc_bloom_u32_num_bits(void* filter);
+WC_API uint64_t wc_bloom_u32_num_slots(void* filter);
+WC_API uint64_t wc_bloom_u32_num_blocks(void* filter);
+WC_API uint64_t wc_bloom_u32_k(void* filter);
+WC_API uint32_t wc_bloom_u32_seed(void* filter);
+WC_API uint32_t wc_bloom_u32_slot_bits(void* filter);
+WC_API uint32_t wc_bloom_u32_cg_size(void* filter);
+WC_API uint64_t wc_bloom_u32_bytes_total(void* filter);
+WC_API double wc_bloom_u32_fpr(void* filter, uint64_t n);
+
+/* BloomFilter<uint64_t> */
+WC_API void* wc_bloom_u64_create(uint64_t num_bits, uint64_t k, uint64_t seed, uint32_t slot_bits, uint32_t cg_size);
+WC_API void wc_bloom_u64_destroy(void* filter);
+WC_API int wc_bloom_u64_init(void* filter, void* stream);
+WC_API int wc_bloom_u64_init_seed(void* filter, uint64_t seed, void* stream);
+WC_API int wc_bloom_u64_insert(void* filter, const uint64_t* keys_in, uint64_t num_in, void* stream);
+WC_API int wc_bloom_u64_retrieve(void* filter, const uint64_t* keys_in, uint64_t num_in, bool* flags_out, void* stream);
+WC_API int wc_bloom_u64_insert_and_query(void* filter, const uint64_t* keys_in, uint64_t num_in, bool* flags_out, void* stream);
+WC_API int wc_bloom_u64_copy_words(void* filter, void* words_out, void* stream);
+WC_API uint64_t wc_bloom_u64_num_bits(void* filter);
+WC_API uint64_t wc_bloom_u64_num_slots(void* filter);
+WC_API uint64_t wc_bloom_u64_num_blocks(void* filter);
+WC_API uint64_t wc_bloom_u64_k(void* filter);
+WC_API uint64_t wc_bloom_u64_seed(void* filter);
+WC_API uint32_t wc_bloom_u64_slot_bits(void* filter);
+WC_API uint32_t wc_bloom_u64_cg_size(void* filter);
+WC_API uint64_t wc_bloom_u64_bytes_total(void* filter);
+WC_API double wc_bloom_u64_fpr(void* filter, uint64_t n);
+
+/* ------------------------------------------------------------------------
  * Synthetic inputs (the role of include/warpcore/random_distributions.cuh:
  * uniform 50-135, unique 144-246, zipf 259-326). Counter-based: element i
  * depends only on (seed or offset, i).

@@ -296,3 +296,81 @@ class HashSet:
 
     def cg_size(self) -> int:
         return self._fn("cg_size")(self._h)
+
+
+class BloomFilter:
+    """BloomFilter<K, MurmurHash<K>, Slot, CG> (include/warpcore/bloom_filter.cuh)."""
+
+    def __init__(self, num_bits: int, k: int, key_bits: int = 64, seed: int = DEFAULT_SEED,
+                 slot_bits: int = 64, cg_size: int = 1):
+        self.key_bits = key_bits
+        self._p = f"wc_bloom_u{key_bits}_"
+        self._lib = _capi.lib()
+        self._h = self._fn("create")(num_bits, k, seed, slot_bits, cg_size)
+        if not self._h:
+            raise ValueError(f"unsupported configuration (num_bits={num_bits}, k={k}, "
+                             f"slot_bits={slot_bits}, cg_size={cg_size})")
+
+    def _fn(self, name):
+        return getattr(self._lib, self._p + name)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._fn("destroy")(h)
+
+    def init(self, seed: int | None = None, stream=None):
+        if seed is None:
+            _check(self._fn("init")(self._h, _stream(stream)), "init")
+        else:
+            _check(self._fn("init_seed")(self._h, seed, _stream(stream)), "init")
+
+    def insert(self, keys_in, stream=None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _check(self._fn("insert")(self._h, _ptr(keys_in), keys_in.numel(), _stream(stream)), "insert")
+
+    def retrieve(self, keys_in, flags_out, stream=None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(flags_out, 8, "flags_out")
+        _check(self._fn("retrieve")(self._h, _ptr(keys_in), keys_in.numel(), _ptr(flags_out), _stream(stream)),
+               "retrieve")
+
+    def insert_and_query(self, keys_in, flags_out, stream=None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(flags_out, 8, "flags_out")
+        _check(self._fn("insert_and_query")(self._h, _ptr(keys_in), keys_in.numel(), _ptr(flags_out),
+                                            _stream(stream)), "insert_and_query")
+
+    def words(self, stream=None):
+        """Copy of the bit array as a CUDA tensor of num_slots words."""
+        dt = torch.int64 if self.slot_bits() == 64 else torch.int32
+        out = torch.empty(max(self.num_slots(), 1), dtype=dt, device="cuda")[: self.num_slots()]
+        _check(self._fn("copy_words")(self._h, _ptr(out), _stream(stream)), "copy_words")
+        return out
+
+    def num_bits(self) -> int:
+        return self._fn("num_bits")(self._h)
+
+    def num_slots(self) -> int:
+        return self._fn("num_slots")(self._h)
+
+    def num_blocks(self) -> int:
+        return self._fn("num_blocks")(self._h)
+
+    def k(self) -> int:
+        return self._fn("k")(self._h)
+
+    def seed(self) -> int:
+        return self._fn("seed")(self._h)
+
+    def slot_bits(self) -> int:
+        return self._fn("slot_bits")(self._h)
+
+    def cg_size(self) -> int:
+        return self._fn("cg_size")(self._h)
+
+    def bytes_total(self) -> int:
+        return self._fn("bytes_total")(self._h)
+
+    def fpr(self, n: int) -> float:
+        return self._fn("fpr")(self._h, n)
