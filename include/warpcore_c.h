@@ -296,6 +296,143 @@ WC_API uint64_t wc_bloom_u64_bytes_total(void* filter);
 WC_API double wc_bloom_u64_fpr(void* filter, uint64_t n);
 
 /* ------------------------------------------------------------------------
+ * MultiValueHashTable<K, V> (include/warpcore/multi_value_hash_table.cuh)
+ *   create       constructor :93-123 (max_values_per_key, no_init); cg_size
+ *                1..64 power of two (0 = 8, the reference default)
+ *   init         :160-177 (empties the table, zeroes the key counter)
+ *   insert       :297-319 (device member :200-289); status per pair: none for
+ *                a key's first value, duplicate_key for later ones,
+ *                duplicate_key|max_values_for_key_reached when refused,
+ *                probing_length_exceeded, invalid_key
+ *   retrieve     two-pass retrieve :370-510: end_offsets_out[i] = inclusive
+ *                prefix sum of the counts, begin_offsets_out[i] = its
+ *                exclusive sum, values of keys_in[i] at
+ *                values_out[begin[i] .. end[i]) in probe order, *num_out =
+ *                total; values_out == NULL is a dry run: end_offsets_out
+ *                gets the raw counts, status dry_run
+ *   num_values   :766-793 per-key counts (num_per_key_out may be NULL) and
+ *                their sum
+ *   retrieve_all_keys :562-600 each distinct key once (order unspecified)
+ *   retrieve_all :524-548 retrieve_all_keys, then retrieve for those keys
+ *   num_keys :721-731, size = num_values :809-834, load_factor,
+ *   storage_density :851-862, peek_status/pop_status :884-925
+ * ---------------------------------------------------------------------- */
+/* MultiValueHashTable<uint32_t, uint32_t> */
+WC_API void* wc_mvht_u32u32_create(uint64_t min_capacity, uint32_t seed, uint64_t max_values_per_key, int no_init, uint32_t cg_size);
+WC_API void wc_mvht_u32u32_destroy(void* table);
+WC_API int wc_mvht_u32u32_init(void* table, void* stream);
+WC_API int wc_mvht_u32u32_insert(void* table, const uint32_t* keys_in, const uint32_t* values_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_mvht_u32u32_retrieve(void* table, const uint32_t* keys_in, uint64_t num_in, uint64_t* begin_offsets_out, uint64_t* end_offsets_out, uint32_t* values_out, uint64_t* num_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_mvht_u32u32_num_values(void* table, const uint32_t* keys_in, uint64_t num_in, uint64_t* num_out, uint64_t* num_per_key_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_mvht_u32u32_retrieve_all_keys(void* table, uint32_t* keys_out, uint64_t* num_out, void* stream);
+WC_API int wc_mvht_u32u32_retrieve_all(void* table, uint32_t* keys_out, uint64_t* num_keys_out, uint64_t* begin_offsets_out, uint64_t* end_offsets_out, uint32_t* values_out, uint64_t* num_values_out, void* stream);
+WC_API uint64_t wc_mvht_u32u32_num_keys(void* table, void* stream);
+WC_API uint64_t wc_mvht_u32u32_size(void* table, void* stream);
+WC_API float wc_mvht_u32u32_load_factor(void* table, void* stream);
+WC_API float wc_mvht_u32u32_storage_density(void* table, void* stream);
+WC_API uint64_t wc_mvht_u32u32_capacity(void* table);
+WC_API uint64_t wc_mvht_u32u32_bytes_total(void* table);
+WC_API uint64_t wc_mvht_u32u32_max_values_per_key(void* table);
+WC_API int wc_mvht_u32u32_is_initialized(void* table);
+WC_API uint32_t wc_mvht_u32u32_peek_status(void* table, void* stream);
+WC_API uint32_t wc_mvht_u32u32_pop_status(void* table, void* stream);
+WC_API uint32_t wc_mvht_u32u32_cg_size(void* table);
+
+/* MultiValueHashTable<uint64_t, uint64_t> */
+WC_API void* wc_mvht_u64u64_create(uint64_t min_capacity, uint64_t seed, uint64_t max_values_per_key, int no_init, uint32_t cg_size);
+WC_API void wc_mvht_u64u64_destroy(void* table);
+WC_API int wc_mvht_u64u64_init(void* table, void* stream);
+WC_API int wc_mvht_u64u64_insert(void* table, const uint64_t* keys_in, const uint64_t* values_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_mvht_u64u64_retrieve(void* table, const uint64_t* keys_in, uint64_t num_in, uint64_t* begin_offsets_out, uint64_t* end_offsets_out, uint64_t* values_out, uint64_t* num_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_mvht_u64u64_num_values(void* table, const uint64_t* keys_in, uint64_t num_in, uint64_t* num_out, uint64_t* num_per_key_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_mvht_u64u64_retrieve_all_keys(void* table, uint64_t* keys_out, uint64_t* num_out, void* stream);
+WC_API int wc_mvht_u64u64_retrieve_all(void* table, uint64_t* keys_out, uint64_t* num_keys_out, uint64_t* begin_offsets_out, uint64_t* end_offsets_out, uint64_t* values_out, uint64_t* num_values_out, void* stream);
+WC_API uint64_t wc_mvht_u64u64_num_keys(void* table, void* stream);
+WC_API uint64_t wc_mvht_u64u64_size(void* table, void* stream);
+WC_API float wc_mvht_u64u64_load_factor(void* table, void* stream);
+WC_API float wc_mvht_u64u64_storage_density(void* table, void* stream);
+WC_API uint64_t wc_mvht_u64u64_capacity(void* table);
+WC_API uint64_t wc_mvht_u64u64_bytes_total(void* table);
+WC_API uint64_t wc_mvht_u64u64_max_values_per_key(void* table);
+WC_API int wc_mvht_u64u64_is_initialized(void* table);
+WC_API uint32_t wc_mvht_u64u64_peek_status(void* table, void* stream);
+WC_API uint32_t wc_mvht_u64u64_pop_status(void* table, void* stream);
+WC_API uint32_t wc_mvht_u64u64_cg_size(void* table);
+
+/* ------------------------------------------------------------------------
+ * BucketListHashTable<K, V> (include/warpcore/bucket_list_hash_table.cuh)
+ * with storage::multi_value::BucketListStore<V> (storage.cuh:656-1454)
+ *   create       constructor :106-125: key_capacity (minimum, capacity rule
+ *                applied), value_capacity (pool cells), grow_factor >= 1,
+ *                1 <= min_bucket_size <= max_bucket_size <= 1023,
+ *                max_values_per_key (clamped to 2^20 - 1); cg_size 0 = 8.
+ *                An invalid pool configuration leaves the table
+ *                uninitialised (status invalid_configuration|not_initialized)
+ *   init         :155-171 (also empties the value pool)
+ *   insert       status per pair = key-table status (none for a new key,
+ *                duplicate_key for a known one) + append status
+ *                (out_of_memory, max_values_for_key_reached)
+ *   retrieve / num_values / retrieve_all_keys / retrieve_all: as for
+ *                MultiValueHashTable; values of a key come tail bucket first
+ *   total_values :805-822 (num_values() of the whole table)
+ *   key_load_factor / value_load_factor / storage_density /
+ *   relative_storage_density / key_capacity / value_capacity :612-737
+ *   value_cells_used  pool cells handed out so far (no reference
+ *                counterpart; value_load_factor * value_capacity)
+ * ---------------------------------------------------------------------- */
+/* BucketListHashTable<uint32_t, uint32_t> */
+WC_API void* wc_bucketlist_u32u32_create(uint64_t key_capacity, uint64_t value_capacity, uint32_t seed, float grow_factor, uint64_t min_bucket_size, uint64_t max_bucket_size, uint64_t max_values_per_key, int no_init, uint32_t cg_size);
+WC_API void wc_bucketlist_u32u32_destroy(void* table);
+WC_API int wc_bucketlist_u32u32_init(void* table, void* stream);
+WC_API int wc_bucketlist_u32u32_init_seed(void* table, uint32_t seed, void* stream);
+WC_API int wc_bucketlist_u32u32_insert(void* table, const uint32_t* keys_in, const uint32_t* values_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_bucketlist_u32u32_retrieve(void* table, const uint32_t* keys_in, uint64_t num_in, uint64_t* begin_offsets_out, uint64_t* end_offsets_out, uint32_t* values_out, uint64_t* num_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_bucketlist_u32u32_num_values(void* table, const uint32_t* keys_in, uint64_t num_in, uint64_t* num_out, uint64_t* num_per_key_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API uint64_t wc_bucketlist_u32u32_total_values(void* table, void* stream);
+WC_API int wc_bucketlist_u32u32_retrieve_all_keys(void* table, uint32_t* keys_out, uint64_t* num_out, void* stream);
+WC_API int wc_bucketlist_u32u32_retrieve_all(void* table, uint32_t* keys_out, uint64_t* num_keys_out, uint64_t* begin_offsets_out, uint64_t* end_offsets_out, uint32_t* values_out, uint64_t* num_values_out, void* stream);
+WC_API uint64_t wc_bucketlist_u32u32_num_keys(void* table, void* stream);
+WC_API float wc_bucketlist_u32u32_key_load_factor(void* table, void* stream);
+WC_API float wc_bucketlist_u32u32_value_load_factor(void* table, void* stream);
+WC_API float wc_bucketlist_u32u32_storage_density(void* table, void* stream);
+WC_API float wc_bucketlist_u32u32_relative_storage_density(void* table, void* stream);
+WC_API uint64_t wc_bucketlist_u32u32_key_capacity(void* table);
+WC_API uint64_t wc_bucketlist_u32u32_value_capacity(void* table);
+WC_API uint64_t wc_bucketlist_u32u32_value_cells_used(void* table, void* stream);
+WC_API uint64_t wc_bucketlist_u32u32_bytes_total(void* table);
+WC_API uint64_t wc_bucketlist_u32u32_max_values_per_key(void* table);
+WC_API int wc_bucketlist_u32u32_is_initialized(void* table);
+WC_API uint32_t wc_bucketlist_u32u32_peek_status(void* table, void* stream);
+WC_API uint32_t wc_bucketlist_u32u32_pop_status(void* table, void* stream);
+WC_API uint32_t wc_bucketlist_u32u32_cg_size(void* table);
+
+/* BucketListHashTable<uint64_t, uint64_t> */
+WC_API void* wc_bucketlist_u64u64_create(uint64_t key_capacity, uint64_t value_capacity, uint64_t seed, float grow_factor, uint64_t min_bucket_size, uint64_t max_bucket_size, uint64_t max_values_per_key, int no_init, uint32_t cg_size);
+WC_API void wc_bucketlist_u64u64_destroy(void* table);
+WC_API int wc_bucketlist_u64u64_init(void* table, void* stream);
+WC_API int wc_bucketlist_u64u64_init_seed(void* table, uint64_t seed, void* stream);
+WC_API int wc_bucketlist_u64u64_insert(void* table, const uint64_t* keys_in, const uint64_t* values_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_bucketlist_u64u64_retrieve(void* table, const uint64_t* keys_in, uint64_t num_in, uint64_t* begin_offsets_out, uint64_t* end_offsets_out, uint64_t* values_out, uint64_t* num_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_bucketlist_u64u64_num_values(void* table, const uint64_t* keys_in, uint64_t num_in, uint64_t* num_out, uint64_t* num_per_key_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API uint64_t wc_bucketlist_u64u64_total_values(void* table, void* stream);
+WC_API int wc_bucketlist_u64u64_retrieve_all_keys(void* table, uint64_t* keys_out, uint64_t* num_out, void* stream);
+WC_API int wc_bucketlist_u64u64_retrieve_all(void* table, uint64_t* keys_out, uint64_t* num_keys_out, uint64_t* begin_offsets_out, uint64_t* end_offsets_out, uint64_t* values_out, uint64_t* num_values_out, void* stream);
+WC_API uint64_t wc_bucketlist_u64u64_num_keys(void* table, void* stream);
+WC_API float wc_bucketlist_u64u64_key_load_factor(void* table, void* stream);
+WC_API float wc_bucketlist_u64u64_value_load_factor(void* table, void* stream);
+WC_API float wc_bucketlist_u64u64_storage_density(void* table, void* stream);
+WC_API float wc_bucketlist_u64u64_relative_storage_density(void* table, void* stream);
+WC_API uint64_t wc_bucketlist_u64u64_key_capacity(void* table);
+WC_API uint64_t wc_bucketlist_u64u64_value_capacity(void* table);
+WC_API uint64_t wc_bucketlist_u64u64_value_cells_used(void* table, void* stream);
+WC_API uint64_t wc_bucketlist_u64u64_bytes_total(void* table);
+WC_API uint64_t wc_bucketlist_u64u64_max_values_per_key(void* table);
+WC_API int wc_bucketlist_u64u64_is_initialized(void* table);
+WC_API uint32_t wc_bucketlist_u64u64_peek_status(void* table, void* stream);
+WC_API uint32_t wc_bucketlist_u64u64_pop_status(void* table, void* stream);
+WC_API uint32_t wc_bucketlist_u64u64_cg_size(void* table);
+
+/* ------------------------------------------------------------------------
  * Synthetic inputs (the role of include/warpcore/random_distributions.cuh:
  * uniform 50-135, unique 144-246, zipf 259-326). Counter-based: element i
  * depends only on (seed or offset, i).

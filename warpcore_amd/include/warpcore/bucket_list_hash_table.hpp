@@ -1,0 +1,752 @@
+// BucketListHashTable: key -> growing list of value buckets. Drop-in for
+// include/warpcore/bucket_list_hash_table.cuh with the value store
+// storage::multi_value::BucketListStore (storage.cuh:656-1454).
+//
+// Keys live in a SingleValueHashTable<Key, handle>; a handle is one 64-bit
+// word {state: 2 bits, bucket index, value count, values in the tail
+// bucket} updated only by CAS/exchange. Values live in a pool of 8-byte
+// cells handed out by an atomicAdd on next_free. A key's first bucket holds
+// min_bucket_size values and has no header; every later bucket is a header
+// cell {previous bucket, bucket size} followed by its values, each bucket
+// ceil(previous size * grow_factor) cells long, capped at max_bucket_size.
+//
+// Append (storage.cuh:1113-1284): an uninitialised handle or a full tail
+// bucket is first switched to `blocking` by a CAS; the winner allocates the
+// bucket, writes the value and publishes the new handle with an exchange;
+// other appenders to the same key re-read the handle until it is no longer
+// blocking. The winner's allocation path contains no wait, so a blocked
+// handle is always released; the re-read loop is also bounded and returns
+// unknown_error if that bound is ever reached. Otherwise the CAS bumps the
+// count and the tail, and the winner stores the value in its cell.
+//
+// Values of a key come back tail bucket first (storage.cuh:1292-1345). A
+// handle left `full` by an out-of-memory append still yields the values it
+// holds; the reference yields none for it while reporting their count.
+#pragma once
+
+#include "base.hpp"
+#include "defaults.hpp"
+#include "engine.hpp"
+#include "multi_value_hash_table.hpp"
+#include "scan.hpp"
+#include "single_value_hash_table.hpp"
+#include "status.hpp"
+#include "tile.hpp"
+
+#include <cmath>
+
+namespace warpcore {
+namespace storage {
+namespace multi_value {
+
+namespace detail {
+
+enum class LinkedListState : std::uint32_t { uninitialized = 0, initialized = 1, blocking = 2, full = 3 };
+
+// One 64-bit word: state | index << 2 | count << (2 + IB) | tail << (2 + IB + CB).
+template<index_t IB, index_t CB, index_t SB>
+struct alignas(8) BucketListHandle {
+    std::uint64_t pack;
+
+    static constexpr std::uint64_t mask(index_t bits) noexcept {
+        return bits >= 64 ? ~std::uint64_t(0) : (std::uint64_t(1) << bits) - 1;
+    }
+    WC_HD static constexpr index_t max_bucket_index() noexcept { return mask(IB); }
+    WC_HD static constexpr index_t max_value_count() noexcept { return mask(CB); }
+    WC_HD static constexpr index_t max_bucket_size() noexcept { return mask(SB); }
+
+    WC_HD static constexpr BucketListHandle make(LinkedListState state, index_t index, index_t count,
+                                                 index_t tail) noexcept {
+        return BucketListHandle{std::uint64_t(state) | ((std::uint64_t(index) & mask(IB)) << 2) |
+                                ((std::uint64_t(count) & mask(CB)) << (2 + IB)) |
+                                ((std::uint64_t(tail) & mask(SB)) << (2 + IB + CB))};
+    }
+    WC_HD constexpr LinkedListState state() const noexcept { return LinkedListState(pack & 3); }
+    WC_HD constexpr index_t bucket_index() const noexcept { return (pack >> 2) & mask(IB); }
+    WC_HD constexpr index_t value_count() const noexcept { return (pack >> (2 + IB)) & mask(CB); }
+    WC_HD constexpr index_t num_values_tail() const noexcept { return (pack >> (2 + IB + CB)) & mask(SB); }
+    WC_HD constexpr bool is_uninitialized() const noexcept { return state() == LinkedListState::uninitialized; }
+    WC_HD constexpr bool is_initialized() const noexcept { return state() == LinkedListState::initialized; }
+    WC_HD constexpr bool is_blocking() const noexcept { return state() == LinkedListState::blocking; }
+    WC_HD constexpr bool is_full() const noexcept { return state() == LinkedListState::full; }
+};
+
+// A pool cell: a value, or the header {previous bucket, bucket size} of a bucket.
+template<class Value, index_t IB, index_t SB>
+union alignas(alignof(Value) > 8 ? alignof(Value) : 8) Bucket {
+    Value value;
+    std::uint64_t info;
+
+    WC_HD static constexpr std::uint64_t make_info(index_t previous, index_t size) noexcept {
+        return (std::uint64_t(previous) & BucketListHandle<IB, 1, SB>::mask(IB)) |
+               ((std::uint64_t(size) & BucketListHandle<IB, 1, SB>::mask(SB)) << IB);
+    }
+    WC_HD static constexpr index_t previous_of(std::uint64_t info) noexcept {
+        return info & BucketListHandle<IB, 1, SB>::mask(IB);
+    }
+    WC_HD static constexpr index_t size_of(std::uint64_t info) noexcept {
+        return (info >> IB) & BucketListHandle<IB, 1, SB>::mask(SB);
+    }
+};
+
+// Device half of the store, passed to kernels by value.
+template<class Value, index_t IB, index_t CB, index_t SB>
+struct BucketListStoreView {
+    using handle_type = BucketListHandle<IB, CB, SB>;
+    using bucket_type = Bucket<Value, IB, SB>;
+    using value_type = Value;
+    // re-reads of a blocked handle before an append gives up (unknown_error)
+    static constexpr std::uint32_t kMaxBlockedReads = 1u << 24;
+
+    bucket_type* buckets;
+    index_t* next_free;
+    index_t capacity;
+    float grow;
+    index_t min_size;
+    index_t max_size;
+
+    WC_HD static constexpr index_t head() noexcept { return handle_type::max_bucket_index(); }
+
+    WC_D Status append(handle_type* handle, const Value& value, index_t max_values_per_key) const noexcept {
+        using warpcore::detail::atomic_add;
+        using warpcore::detail::atomic_cas;
+        using warpcore::detail::atomic_exch;
+        using LS = detail::LinkedListState;
+        std::uint64_t* hp = &handle->pack;
+        for (std::uint32_t reads = 0; reads < kMaxBlockedReads; ++reads) {
+            const handle_type cur{warpcore::detail::atomic_load_relaxed(hp)};
+            if (cur.is_blocking()) continue;
+            if (cur.is_uninitialized()) {
+                const handle_type blocked = handle_type::make(LS::blocking, head(), 0, 0);
+                if (atomic_cas(hp, cur.pack, blocked.pack) != cur.pack) continue;
+                const index_t alloc = atomic_add(next_free, min_size);
+                if (alloc + min_size <= capacity) {
+                    buckets[alloc].value = value;
+                    __threadfence();
+                    atomic_exch(hp, handle_type::make(LS::initialized, alloc, 1, 1).pack);
+                    return Status::none();
+                }
+                atomic_exch(hp, handle_type::make(LS::full, head(), 0, 0).pack);
+                return Status::out_of_memory();
+            }
+            if (cur.is_full()) return Status::out_of_memory();
+            if (cur.value_count() == max_values_per_key) return Status::max_values_for_key_reached();
+            const index_t tail_size =
+                cur.value_count() <= min_size
+                    ? min_size
+                    : bucket_type::size_of(warpcore::detail::atomic_load_relaxed(&buckets[cur.bucket_index()].info));
+            if (cur.num_values_tail() == tail_size) {
+                const handle_type blocked =
+                    handle_type::make(LS::blocking, cur.bucket_index(), cur.value_count(), cur.num_values_tail());
+                if (atomic_cas(hp, cur.pack, blocked.pack) != cur.pack) continue;
+                const index_t new_size =
+                    index_t(fminf(float(max_size), ceilf(float(tail_size) * grow)));
+                const index_t alloc = atomic_add(next_free, new_size + 1);
+                if (alloc + new_size + 1 <= capacity) {
+                    buckets[alloc + 1].value = value;
+                    atomic_exch(&buckets[alloc].info, bucket_type::make_info(cur.bucket_index(), new_size));
+                    __threadfence();
+                    atomic_exch(hp, handle_type::make(LS::initialized, alloc, cur.value_count() + 1, 1).pack);
+                    return Status::none();
+                }
+                atomic_exch(hp, handle_type::make(LS::full, cur.bucket_index(), cur.value_count(),
+                                                  cur.num_values_tail())
+                                    .pack);
+                return Status::out_of_memory();
+            }
+            const handle_type next = handle_type::make(LS::initialized, cur.bucket_index(), cur.value_count() + 1,
+                                                       cur.num_values_tail() + 1);
+            if (atomic_cas(hp, cur.pack, next.pack) == cur.pack) {
+                const index_t j = cur.value_count() + 1 <= min_size ? cur.num_values_tail()
+                                                                                : cur.num_values_tail() + 1;
+                buckets[cur.bucket_index() + j].value = value;
+                return Status::none();
+            }
+        }
+        return Status::unknown_error();
+    }
+
+    // f(value, j) for j < count, lanes of `group` striding over the values
+    template<class Func, unsigned G>
+    WC_D void for_each(Func&& f, const handle_type& handle, const tile<G>& group) const noexcept {
+        for_each_strided(f, handle, group.thread_rank(), G);
+    }
+
+    // Lane `rank` of `group_size` lanes takes positions rank, rank + group_size,
+    // ... of every bucket; all lanes walk the chain together, so each knows
+    // which bucket it is in. (The reference advances lane-local and global
+    // indices separately, storage.cuh:1310-1343, which for groups of more
+    // than one lane mistakes an earlier bucket for the first, headerless one
+    // and then follows a value as a link.)
+    template<class Func>
+    WC_D void for_each_strided(Func& f, const handle_type& handle, index_t rank, index_t group_size) const
+        noexcept {
+        if ((!handle.is_initialized() && !handle.is_full()) || handle.bucket_index() == head()) return;
+        const index_t count = handle.value_count();
+        const index_t tail = handle.num_values_tail();
+        index_t bucket = handle.bucket_index();
+        const index_t offset = count <= min_size ? 0 : 1;
+        for (index_t p = rank; p < tail; p += group_size) f(buckets[bucket + offset + p].value, p);
+        index_t base = tail;   // values before the current bucket, in walk order
+        while (base < count) {
+            bucket = bucket_type::previous_of(buckets[bucket].info);
+            const bool last = base >= count - min_size;   // the first, headerless bucket
+            const index_t size = last ? min_size : bucket_type::size_of(buckets[bucket].info);
+            const index_t off = last ? 0 : 1;
+            if (size == 0 || bucket == head() || bucket + off + size > capacity) return;   // not on a well-formed chain
+            for (index_t p = rank; p < size; p += group_size) f(buckets[bucket + off + p].value, base + p);
+            base += size;
+        }
+    }
+};
+
+}  // namespace detail
+
+template<class Value, index_t BucketIndexBits = 32, index_t ValueCounterBits = 20, index_t BucketSizeBits = 10>
+class BucketListStore {
+    static_assert(checks::is_valid_value_type<Value>(), "Value type must be trivially copyable.");
+    static_assert(BucketIndexBits + ValueCounterBits + BucketSizeBits + 2 <= 64,
+                  "Too many bits for bucket index and value counter and bucket size.");
+    static_assert(BucketIndexBits + BucketSizeBits <= 64, "bucket header does not fit 64 bits");
+
+public:
+    using value_type = Value;
+    using handle_type = detail::BucketListHandle<BucketIndexBits, ValueCounterBits, BucketSizeBits>;
+    using bucket_type = detail::Bucket<Value, BucketIndexBits, BucketSizeBits>;
+    using view_type = detail::BucketListStoreView<Value, BucketIndexBits, ValueCounterBits, BucketSizeBits>;
+    using index_type = index_t;
+    using status_type = Status;
+    using tag = tags::dynamic_value_storage;
+
+    WC_HD static constexpr index_type bucket_index_bits() noexcept { return BucketIndexBits; }
+    WC_HD static constexpr index_type value_counter_bits() noexcept { return ValueCounterBits; }
+    WC_HD static constexpr index_type bucket_size_bits() noexcept { return BucketSizeBits; }
+    WC_HD static constexpr index_type head() noexcept { return handle_type::max_bucket_index(); }
+    WC_HD static constexpr handle_type uninitialized_handle() noexcept {
+        return handle_type::make(detail::LinkedListState::uninitialized, head(), 0, 0);
+    }
+    WC_HD static constexpr index_type size(const handle_type& handle) noexcept { return handle.value_count(); }
+
+    // storage.cuh:989-1029
+    WC_H explicit BucketListStore(index_type max_capacity, float bucket_grow_factor = 1.1f,
+                                  index_type min_bucket_size = 1,
+                                  index_type max_bucket_size = handle_type::max_bucket_size()) noexcept
+        : status_(Status::not_initialized()), capacity_(max_capacity), bucket_grow_factor_(bucket_grow_factor),
+          min_bucket_size_(min_bucket_size), max_bucket_size_(max_bucket_size), buckets_(nullptr),
+          next_free_bucket_(nullptr), is_copy_(false) {
+        if (capacity_ < handle_type::max_bucket_index() && bucket_grow_factor_ >= 1.0f && min_bucket_size_ >= 1 &&
+            max_bucket_size_ >= min_bucket_size_ && max_bucket_size_ <= handle_type::max_bucket_size()) {
+            const std::size_t bytes = sizeof(bucket_type) * capacity_ + sizeof(index_type);
+            if (warpcore::detail::available_gpu_memory() >= bytes &&
+                (capacity_ == 0 || hipMalloc(&buckets_, sizeof(bucket_type) * capacity_) == hipSuccess) &&
+                hipMalloc(&next_free_bucket_, sizeof(index_type)) == hipSuccess) {
+                status_ = Status::none();
+                init();
+            } else {
+                status_ += Status::out_of_memory();
+            }
+        } else {
+            status_ += Status::invalid_configuration();
+        }
+    }
+    WC_HD BucketListStore(const BucketListStore& o) noexcept
+        : status_(o.status_), capacity_(o.capacity_), bucket_grow_factor_(o.bucket_grow_factor_),
+          min_bucket_size_(o.min_bucket_size_), max_bucket_size_(o.max_bucket_size_), buckets_(o.buckets_),
+          next_free_bucket_(o.next_free_bucket_), is_copy_(true) {}
+    WC_H BucketListStore(BucketListStore&& o) noexcept
+        : status_(o.status_), capacity_(o.capacity_), bucket_grow_factor_(o.bucket_grow_factor_),
+          min_bucket_size_(o.min_bucket_size_), max_bucket_size_(o.max_bucket_size_), buckets_(o.buckets_),
+          next_free_bucket_(o.next_free_bucket_), is_copy_(o.is_copy_) {
+        o.is_copy_ = true;
+    }
+    WC_HD ~BucketListStore() noexcept {
+#if !defined(__HIP_DEVICE_COMPILE__)
+        if (!is_copy_) {
+            if (buckets_ != nullptr) (void)hipFree(buckets_);
+            if (next_free_bucket_ != nullptr) (void)hipFree(next_free_bucket_);
+        }
+#endif
+    }
+
+    // storage.cuh:1063-1090: every cell becomes a header {head, min size};
+    // also empties the pool (next_free = 0)
+    WC_H void init(hipStream_t stream = 0) noexcept {
+        if (status_.has_any() || next_free_bucket_ == nullptr) return;
+        if (capacity_ != 0)
+            kernels::fill<std::uint64_t><<<warpcore::detail::grid_for(capacity_, 1, 8), block_size(), 0, stream>>>(
+                reinterpret_cast<std::uint64_t*>(buckets_), capacity_ * (sizeof(bucket_type) / 8),
+                bucket_type::make_info(head(), min_bucket_size_));
+        (void)hipMemsetAsync(next_free_bucket_, 0, sizeof(index_type), stream);
+        status_ = Status::none();
+    }
+
+    WC_D status_type append(handle_type* handle, const value_type& value, index_type max_values_per_key) const
+        noexcept {
+        return view().append(handle, value, max_values_per_key);
+    }
+
+    // f(value, j) for j < size(handle), lanes of `group` striding over the
+    // values (storage.cuh:1292-1345)
+    template<class Func, unsigned G>
+    WC_D void for_each(Func&& f, const handle_type& handle, const tile<G>& group) const noexcept {
+        view().for_each(f, handle, group);
+    }
+    template<class Func>
+    WC_D void for_each(Func&& f, const handle_type& handle) const noexcept {
+        view().for_each(f, handle, tile<1>{});
+    }
+
+    WC_HD view_type view() const noexcept {
+        return view_type{buckets_, next_free_bucket_, capacity_, bucket_grow_factor_, min_bucket_size_,
+                         max_bucket_size_};
+    }
+
+    WC_HD status_type status() const noexcept { return status_; }
+    WC_HD index_type capacity() const noexcept { return capacity_; }
+    WC_H index_type bytes_total() const noexcept { return capacity_ * sizeof(bucket_type) + sizeof(index_type); }
+    WC_H index_type next_free(hipStream_t stream = 0) const noexcept {
+        index_type n = 0;
+        if (next_free_bucket_ == nullptr) return 0;
+        (void)hipMemcpyAsync(&n, next_free_bucket_, sizeof(index_type), hipMemcpyDeviceToHost, stream);
+        (void)hipStreamSynchronize(stream);
+        return n;
+    }
+    // storage.cuh:1370-1380 (next_free counts cells handed out, also those
+    // of failed allocations)
+    WC_H float load_factor(hipStream_t stream = 0) const noexcept {
+        return float(next_free(stream)) / float(capacity());
+    }
+    WC_H index_type bytes_occupied(hipStream_t stream = 0) const noexcept {
+        return next_free(stream) * sizeof(bucket_type);
+    }
+    WC_HD float bucket_grow_factor() const noexcept { return bucket_grow_factor_; }
+    WC_HD index_type min_bucket_size() const noexcept { return min_bucket_size_; }
+    WC_HD index_type max_bucket_size() const noexcept { return max_bucket_size_; }
+    WC_HD bool is_copy() const noexcept { return is_copy_; }
+
+private:
+    status_type status_;
+    index_type capacity_;
+    float bucket_grow_factor_;
+    index_type min_bucket_size_;
+    index_type max_bucket_size_;
+    bucket_type* buckets_;
+    index_type* next_free_bucket_;
+    bool is_copy_;
+};
+
+}  // namespace multi_value
+}  // namespace storage
+
+namespace defaults {
+template<class Value> using value_storage_t = storage::multi_value::BucketListStore<Value>;
+}  // namespace defaults
+
+namespace detail {
+
+// Device state of a bucket-list table: the key table's view plus the store.
+template<class KeyView, class Store>
+struct BucketListView {
+    using key_type = typename KeyView::key_type;
+    using value_type = typename Store::value_type;
+    using handle_type = typename Store::handle_type;
+    KeyView keys;
+    typename Store::view_type store;
+    index_t max_values;
+
+    WC_HD static constexpr index_t cg_size() noexcept { return KeyView::cg_size(); }
+    WC_HD Status* status_word() const noexcept { return keys.status_word(); }
+
+    template<class Join>
+    WC_D Status insert(key_type key_in, const value_type& value_in, const tile<cg_size()>& group,
+                       index_t probing_length, Join& join) const noexcept {
+        Status status = Status::unknown_error();
+        bool published = false;
+        const index_t slot = keys.insert_impl(key_in, status, group, probing_length, join,
+                                              static_cast<const handle_type*>(nullptr), published);
+        if (slot == ~index_t(0)) return status;
+        handle_type* handle = &keys.slots[slot].value;
+        index_t count = 0;
+        if (group.thread_rank() == 0) count = handle_type{atomic_load_relaxed(&handle->pack)}.value_count();
+        count = group.shfl(count, 0);
+        if (count >= max_values) {
+            join(Status::max_values_for_key_reached());
+            return status + Status::max_values_for_key_reached();
+        }
+        status_base_t appended = 0;
+        if (group.thread_rank() == 0) {
+            const Status a = store.append(handle, value_in, max_values);
+            if (a.has_any()) join(a);
+            appended = a.base();
+        }
+        return status + Status::from_base(group.shfl(appended, 0));
+    }
+
+    // handle of key_in from the key table (status none when found)
+    template<class Join>
+    WC_D Status find(key_type key_in, handle_type& handle, const tile<cg_size()>& group, index_t probing_length,
+                     Join& join) const noexcept {
+        return keys.retrieve(key_in, handle, group, probing_length, join);
+    }
+};
+
+}  // namespace detail
+
+namespace kernels {
+
+template<class View, class StatusHandler>
+__global__ __launch_bounds__(256) void bl_insert(const typename View::key_type* __restrict__ keys_in,
+                                                 const typename View::value_type* __restrict__ values_in,
+                                                 index_t num_in, View view, index_t probing_length,
+                                                 typename StatusHandler::base_type* __restrict__ status_out) {
+    constexpr unsigned CG = unsigned(View::cg_size());
+    const tile<CG> group;
+    warpcore::detail::DeferredJoin join;
+    const warpcore::detail::GroupRange<CG> r;
+    for (index_t gid = r.first; gid < num_in; gid += r.stride) {
+        const Status st = view.insert(keys_in[gid], values_in[gid], group, probing_length, join);
+        if (group.thread_rank() == 0) StatusHandler::handle(st, status_out, gid);
+    }
+    join.flush(view.status_word());
+}
+
+// per-key value counts and their total (gpu_engine.cuh:466-504)
+template<class View, class StatusHandler>
+__global__ __launch_bounds__(256) void bl_num_values(const typename View::key_type* __restrict__ keys_in,
+                                                     index_t num_in, index_t* __restrict__ total_out,
+                                                     index_t* __restrict__ num_per_key_out, View view,
+                                                     index_t probing_length,
+                                                     typename StatusHandler::base_type* __restrict__ status_out) {
+    constexpr unsigned CG = unsigned(View::cg_size());
+    const tile<CG> group;
+    warpcore::detail::DeferredJoin join;
+    warpcore::detail::DeferredCount total;
+    const warpcore::detail::GroupRange<CG> r;
+    for (index_t gid = r.first; gid < num_in; gid += r.stride) {
+        typename View::handle_type h{0};
+        const Status st = view.find(keys_in[gid], h, group, probing_length, join);
+        const index_t num = st.has_any() ? 0 : h.value_count();
+        if (group.thread_rank() == 0) {
+            if (num_per_key_out != nullptr) num_per_key_out[gid] = num;
+            total(num);
+            StatusHandler::handle(st, status_out, gid);
+        }
+    }
+    join.flush(view.status_word());
+    total.flush(total_out);
+}
+
+// gather pass of the two-pass retrieve (gpu_engine.cuh:305-350)
+template<class View, class StatusHandler>
+__global__ __launch_bounds__(256) void bl_retrieve(const typename View::key_type* __restrict__ keys_in,
+                                                   index_t num_in, const index_t* __restrict__ begin_in,
+                                                   const index_t* __restrict__ end_in,
+                                                   typename View::value_type* __restrict__ values_out, View view,
+                                                   index_t probing_length,
+                                                   typename StatusHandler::base_type* __restrict__ status_out) {
+    using V = typename View::value_type;
+    constexpr unsigned CG = unsigned(View::cg_size());
+    const tile<CG> group;
+    warpcore::detail::DeferredJoin join;
+    const warpcore::detail::GroupRange<CG> r;
+    for (index_t gid = r.first; gid < num_in; gid += r.stride) {
+        const index_t begin = begin_in[gid];
+        const index_t room = end_in[gid] - begin;
+        V* out = values_out + begin;
+        typename View::handle_type h{0};
+        Status st = view.find(keys_in[gid], h, group, probing_length, join);
+        index_t num = 0;
+        if (!st.has_any()) {
+            view.store.for_each([=](const V& v, index_t j) { if (j < room) out[j] = v; }, h, group);
+            num = h.value_count();
+        }
+        if (group.thread_rank() == 0) {
+            if (num != room) {
+                join(Status::invalid_phase_overlap());
+                st += Status::invalid_phase_overlap();
+            }
+            StatusHandler::handle(st, status_out, gid);
+        }
+    }
+    join.flush(view.status_word());
+}
+
+// sum of the value counts of all keys (bucket_list_hash_table.cuh:805-822)
+template<class View>
+__global__ __launch_bounds__(256) void bl_total_values(index_t* __restrict__ num_out, View view) {
+    const index_t stride = index_t(gridDim.x) * blockDim.x;
+    index_t count = 0;
+    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < view.keys.capacity(); i += stride) {
+        const auto s = view.keys.slot_at(i);
+        if (view.keys.is_valid_key(s.key)) count += s.value.value_count();
+    }
+    count = tile<64>::reduce_add(count);
+    if (lane_id() == 0 && count) warpcore::detail::atomic_add(num_out, count);
+}
+
+}  // namespace kernels
+
+template<class Key, class Value, Key EmptyKey = defaults::empty_key<Key>(),
+         Key TombstoneKey = defaults::tombstone_key<Key>(), class ValueStore = defaults::value_storage_t<Value>,
+         class ProbingScheme = defaults::probing_scheme_t<Key, 8>>
+class BucketListHashTable {
+    static_assert(checks::is_value_storage<ValueStore>(), "not a valid storage type");
+
+public:
+    using handle_type = typename ValueStore::handle_type;
+
+private:
+    using hash_table_type = SingleValueHashTable<Key, handle_type, EmptyKey, TombstoneKey, ProbingScheme>;
+    using value_store_type = ValueStore;
+
+public:
+    using key_type = Key;
+    using value_type = Value;
+    using index_type = index_t;
+    using status_type = Status;
+    using view_type = detail::BucketListView<typename hash_table_type::view_type, ValueStore>;
+
+    WC_HD static constexpr key_type empty_key() noexcept { return EmptyKey; }
+    WC_HD static constexpr key_type tombstone_key() noexcept { return TombstoneKey; }
+    WC_HD static constexpr bool is_valid_key(key_type key) noexcept {
+        return key != empty_key() && key != tombstone_key();
+    }
+    WC_HD static constexpr index_type cg_size() noexcept { return hash_table_type::cg_size(); }
+    WC_HD static constexpr index_type max_bucket_size() noexcept { return handle_type::max_bucket_size(); }
+
+    // bucket_list_hash_table.cuh:106-125
+    WC_H explicit BucketListHashTable(index_type key_capacity, index_type value_capacity,
+                                      key_type seed = defaults::seed<key_type>(), float grow_factor = 1.1f,
+                                      index_type min_bucket_size = 1, index_type max_bucket_size_ = max_bucket_size(),
+                                      index_type max_values_per_key = handle_type::max_value_count(),
+                                      bool no_init = false) noexcept
+        : hash_table_(key_capacity, seed, true),
+          value_store_(value_capacity, grow_factor, min_bucket_size, max_bucket_size_),
+          max_values_per_key_(std::min(max_values_per_key, handle_type::max_value_count())), is_copy_(false),
+          scan_parts_(nullptr) {
+        if (hipMalloc(&scan_parts_, sizeof(index_type) * warpcore::detail::kScanParts) != hipSuccess) {
+            scan_parts_ = nullptr;
+            join_status(Status::out_of_memory());
+        }
+        join_status(value_store_.status());
+        if (!no_init) init(seed);
+    }
+    WC_HD BucketListHashTable(const BucketListHashTable& o) noexcept
+        : hash_table_(o.hash_table_), value_store_(o.value_store_), max_values_per_key_(o.max_values_per_key_),
+          is_copy_(true), scan_parts_(o.scan_parts_) {}
+    WC_H BucketListHashTable(BucketListHashTable&& o) noexcept
+        : hash_table_(std::move(o.hash_table_)), value_store_(std::move(o.value_store_)),
+          max_values_per_key_(o.max_values_per_key_), is_copy_(o.is_copy_), scan_parts_(o.scan_parts_) {
+        o.is_copy_ = true;
+    }
+    WC_HD ~BucketListHashTable() noexcept {
+#if !defined(__HIP_DEVICE_COMPILE__)
+        if (!is_copy_ && scan_parts_ != nullptr) (void)hipFree(scan_parts_);
+#endif
+    }
+
+    // :155-171
+    WC_H void init(key_type seed, hipStream_t stream = 0) noexcept {
+        const status_type status = hash_table_.peek_status(stream);
+        if (!status.has_not_initialized()) {
+            hash_table_.init(seed, stream);
+            value_store_.init(stream);
+            hash_table_.table_.init_pairs(empty_key(), ValueStore::uninitialized_handle(), stream);
+        }
+    }
+    WC_H void init(hipStream_t stream = 0) noexcept { init(hash_table_.seed(), stream); }
+
+    // ---- device calls
+    WC_D status_type insert(key_type key_in, const value_type& value_in, const tile<cg_size()>& group,
+                            index_type probing_length = defaults::probing_length()) const noexcept {
+        warpcore::detail::ImmediateJoin join{hash_table_.status_};
+        return view().insert(key_in, value_in, group, probing_length, join);
+    }
+    WC_D status_type retrieve(key_type key_in, value_type* values_out, index_type& num_out,
+                              const tile<cg_size()>& group,
+                              index_type probing_length = defaults::probing_length()) const noexcept {
+        warpcore::detail::ImmediateJoin join{hash_table_.status_};
+        handle_type h{0};
+        const status_type st = view().find(key_in, h, group, probing_length, join);
+        if (!st.has_any()) {
+            value_store_.for_each([=](const value_type& v, index_type j) { values_out[j] = v; }, h, group);
+            num_out = h.value_count();
+        } else {
+            num_out = 0;
+        }
+        return st;
+    }
+    template<class Func>
+    WC_D status_type for_each(Func f, key_type key_in, const tile<cg_size()>& group,
+                              index_type probing_length = defaults::probing_length()) const noexcept {
+        warpcore::detail::ImmediateJoin join{hash_table_.status_};
+        handle_type h{0};
+        const status_type st = view().find(key_in, h, group, probing_length, join);
+        if (!st.has_any()) value_store_.for_each(f, h, group);
+        return st;
+    }
+    WC_D status_type num_values(key_type key_in, index_type& num_out, const tile<cg_size()>& group,
+                                index_type probing_length = defaults::probing_length()) const noexcept {
+        warpcore::detail::ImmediateJoin join{hash_table_.status_};
+        handle_type h{0};
+        const status_type st = view().find(key_in, h, group, probing_length, join);
+        num_out = st.has_any() ? 0 : h.value_count();
+        return st;
+    }
+
+    // ---- host calls
+    // :188-292 (status per pair = key-table status + append status)
+    template<class StatusHandler = defaults::status_handler_t>
+    WC_H void insert(const key_type* keys_in, const value_type* values_in, index_type num_in,
+                     hipStream_t stream = 0, index_type probing_length = defaults::probing_length(),
+                     typename StatusHandler::base_type* status_out = nullptr) noexcept {
+        static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
+        if (!hash_table_.is_initialized_ || num_in == 0) return;
+        kernels::bl_insert<view_type, StatusHandler>
+            <<<warpcore::detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(
+                keys_in, values_in, num_in, view(), probing_length, status_out);
+        if (stream == 0) (void)hipStreamSynchronize(stream);
+    }
+
+    // :333-424, the same two passes as MultiValueHashTable::retrieve
+    template<class StatusHandler = defaults::status_handler_t>
+    WC_H void retrieve(const key_type* keys_in, index_type num_in, index_type* begin_offsets_out,
+                       index_type* end_offsets_out, value_type* values_out, index_type& num_out,
+                       hipStream_t stream = 0, index_type probing_length = defaults::probing_length(),
+                       typename StatusHandler::base_type* status_out = nullptr) const noexcept {
+        static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
+        if (!hash_table_.is_initialized_) return;
+        if (num_in > index_type(std::numeric_limits<int>::max())) {
+            join_status(Status::index_overflow(), stream);
+            return;
+        }
+        if (num_in == 0) {
+            num_out = 0;
+            return;
+        }
+        num_values(keys_in, num_in, num_out, end_offsets_out, stream, probing_length);
+        if (values_out != nullptr) {
+            if (scan_parts_ == nullptr) {
+                join_status(Status::out_of_memory(), stream);
+                num_out = 0;
+                return;
+            }
+            warpcore::detail::device_scan(end_offsets_out, end_offsets_out, num_in, scan_parts_, true, stream);
+            kernels::shift_offsets<index_type><<<warpcore::detail::grid_for(num_in, 1, 8), block_size(), 0, stream>>>(
+                begin_offsets_out, end_offsets_out, num_in);
+            kernels::bl_retrieve<view_type, StatusHandler>
+                <<<warpcore::detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(
+                    keys_in, num_in, begin_offsets_out, end_offsets_out, values_out, view(), probing_length,
+                    status_out);
+        } else {
+            if (status_out != nullptr)
+                kernels::handle_all<StatusHandler>
+                    <<<warpcore::detail::grid_for(num_in, 1, 8), block_size(), 0, stream>>>(Status::dry_run(),
+                                                                                             status_out, num_in);
+            join_status(Status::dry_run(), stream);
+        }
+        if (stream == 0) (void)hipStreamSynchronize(stream);
+    }
+
+    // :460-488; num_values_out is index_type& (see MultiValueHashTable)
+    WC_H void retrieve_all(key_type* keys_out, index_type& num_keys_out, index_type* begin_offsets_out,
+                           index_type* end_offsets_out, value_type* values_out, index_type& num_values_out,
+                           hipStream_t stream = 0) const noexcept {
+        retrieve_all_keys(keys_out, num_keys_out, stream);
+        if (keys_out != nullptr)
+            retrieve(keys_out, num_keys_out, begin_offsets_out, end_offsets_out, values_out, num_values_out,
+                     stream);
+        if (stream == 0) (void)hipStreamSynchronize(stream);
+    }
+
+    // :495-527; order unspecified
+    WC_H void retrieve_all_keys(key_type* keys_out, index_type& num_out, hipStream_t stream = 0) const noexcept {
+        if (!hash_table_.is_initialized_) return;
+        if (keys_out == nullptr) {
+            num_out = hash_table_.size(stream);
+            return;
+        }
+        index_type* tmp = hash_table_.temp_.get();
+        (void)hipMemsetAsync(tmp, 0, sizeof(index_type), stream);
+        kernels::retrieve_all<typename hash_table_type::view_type>
+            <<<warpcore::detail::grid_for(key_capacity(), 1, 8), block_size(), 0, stream>>>(
+                keys_out, static_cast<handle_type*>(nullptr), tmp, hash_table_.view());
+        (void)hipMemcpyAsync(&num_out, tmp, sizeof(index_type), hipMemcpyDeviceToHost, stream);
+        (void)hipStreamSynchronize(stream);
+    }
+
+    // :769-803
+    template<class StatusHandler = defaults::status_handler_t>
+    WC_H void num_values(const key_type* keys_in, index_type num_in, index_type& num_out,
+                         index_type* num_per_key_out = nullptr, hipStream_t stream = 0,
+                         index_type probing_length = defaults::probing_length(),
+                         typename StatusHandler::base_type* status_out = nullptr) const noexcept {
+        static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
+        if (!hash_table_.is_initialized_) return;
+        index_type* tmp = hash_table_.temp_.get();
+        (void)hipMemsetAsync(tmp, 0, sizeof(index_type), stream);
+        if (num_in != 0)
+            kernels::bl_num_values<view_type, StatusHandler>
+                <<<warpcore::detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(
+                    keys_in, num_in, tmp, num_per_key_out, view(), probing_length, status_out);
+        (void)hipMemcpyAsync(&num_out, tmp, sizeof(index_type), hipMemcpyDeviceToHost, stream);
+        (void)hipStreamSynchronize(stream);
+    }
+
+    // :805-822
+    WC_H index_type num_values(hipStream_t stream = 0) const noexcept {
+        if (!hash_table_.is_initialized_) return 0;
+        index_type* tmp = hash_table_.temp_.get();
+        (void)hipMemsetAsync(tmp, 0, sizeof(index_type), stream);
+        kernels::bl_total_values<view_type>
+            <<<warpcore::detail::grid_for(key_capacity(), 1, 8), block_size(), 0, stream>>>(tmp, view());
+        index_type out = 0;
+        (void)hipMemcpyAsync(&out, tmp, sizeof(index_type), hipMemcpyDeviceToHost, stream);
+        (void)hipStreamSynchronize(stream);
+        return out;
+    }
+
+    WC_H float key_load_factor(hipStream_t stream = 0) const noexcept { return hash_table_.load_factor(stream); }
+    WC_H float value_load_factor(hipStream_t stream = 0) const noexcept { return value_store_.load_factor(stream); }
+    WC_H index_type bytes_total() const noexcept { return hash_table_.bytes_total() + value_store_.bytes_total(); }
+    WC_H index_type bytes_keys(hipStream_t stream = 0) const noexcept { return num_keys(stream) * sizeof(key_type); }
+    WC_H index_type bytes_values(hipStream_t stream = 0) const noexcept {
+        return num_values(stream) * sizeof(value_type);
+    }
+    WC_H index_type bytes_payload(hipStream_t stream = 0) const noexcept {
+        return bytes_keys(stream) + bytes_values(stream);
+    }
+    WC_H float storage_density(hipStream_t stream = 0) const noexcept {
+        return float(bytes_payload(stream)) / float(bytes_total());
+    }
+    WC_H float relative_storage_density(hipStream_t stream = 0) const noexcept {
+        const float bytes_hash_table = float(hash_table_.capacity() * (sizeof(key_type) + sizeof(handle_type)));
+        const float bytes_value_store = float(value_store_.bytes_occupied(stream));
+        return float(bytes_payload(stream)) / (bytes_value_store + bytes_hash_table);
+    }
+    WC_HD bool is_initialized() const noexcept { return hash_table_.is_initialized(); }
+    WC_H status_type peek_status(hipStream_t stream = 0) const noexcept { return hash_table_.peek_status(stream); }
+    WC_H status_type pop_status(hipStream_t stream = 0) noexcept { return hash_table_.pop_status(stream); }
+    WC_HD index_type key_capacity() const noexcept { return hash_table_.capacity(); }
+    WC_HD index_type value_capacity() const noexcept { return value_store_.capacity(); }
+    WC_H index_type num_keys(hipStream_t stream = 0) const noexcept { return hash_table_.size(stream); }
+    WC_H index_type value_cells_used(hipStream_t stream = 0) const noexcept { return value_store_.next_free(stream); }
+    WC_HD index_type max_values_per_key() const noexcept { return max_values_per_key_; }
+    WC_HD bool is_copy() const noexcept { return is_copy_; }
+
+    WC_HD view_type view() const noexcept {
+        return view_type{hash_table_.view(), value_store_.view(), max_values_per_key_};
+    }
+
+private:
+    WC_H void join_status(status_type status, hipStream_t stream = 0) const noexcept {
+        hash_table_.join_status(status, stream);
+    }
+    hash_table_type hash_table_;
+    value_store_type value_store_;
+    index_type max_values_per_key_;
+    bool is_copy_;
+    index_type* scan_parts_;   // scratch of the two-pass retrieve's prefix sum
+};
+
+}  // namespace warpcore

@@ -374,3 +374,221 @@ class BloomFilter:
 
     def fpr(self, n: int) -> float:
         return self._fn("fpr")(self._h, n)
+
+
+class MultiValueHashTable:
+    """MultiValueHashTable<K, V> (include/warpcore/multi_value_hash_table.cuh)."""
+
+    _NAMES = {(32, 32): "u32u32", (64, 64): "u64u64"}
+
+    def __init__(self, min_capacity: int, key_bits: int = 32, value_bits: int = 32, seed: int = DEFAULT_SEED,
+                 max_values_per_key: int = (1 << 64) - 1, no_init: bool = False, cg_size: int = 0):
+        self.key_bits, self.value_bits = key_bits, value_bits
+        self._p = f"wc_mvht_{self._NAMES[(key_bits, value_bits)]}_"
+        self._lib = _capi.lib()
+        self._h = self._fn("create")(min_capacity, seed, max_values_per_key, int(no_init), cg_size)
+        if not self._h:
+            raise ValueError(f"unsupported configuration (cg_size={cg_size})")
+
+    def _fn(self, name):
+        return getattr(self._lib, self._p + name)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._fn("destroy")(h)
+
+    def init(self, stream=None):
+        _check(self._fn("init")(self._h, _stream(stream)), "init")
+
+    def insert(self, keys_in, values_in, stream=None, probing_length=DEFAULT_PROBING_LENGTH, status_out=None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(values_in, self.value_bits, "values_in")
+        _width(status_out, 32, "status_out")
+        _check(self._fn("insert")(self._h, _ptr(keys_in), _ptr(values_in), keys_in.numel(), _stream(stream),
+                                  probing_length, _ptr(status_out)), "insert")
+
+    def retrieve(self, keys_in, begin_offsets_out, end_offsets_out, values_out, stream=None,
+                 probing_length=DEFAULT_PROBING_LENGTH, status_out=None) -> int:
+        """Two-pass retrieve; returns the total number of values. values_out=None is a dry run."""
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(values_out, self.value_bits, "values_out")
+        _width(begin_offsets_out, 64, "begin_offsets_out")
+        _width(end_offsets_out, 64, "end_offsets_out")
+        n = ctypes.c_uint64(0)
+        _check(self._fn("retrieve")(self._h, _ptr(keys_in), keys_in.numel(), _ptr(begin_offsets_out),
+                                    _ptr(end_offsets_out), _ptr(values_out), ctypes.byref(n), _stream(stream),
+                                    probing_length, _ptr(status_out)), "retrieve")
+        return n.value
+
+    def num_values(self, keys_in, num_per_key_out=None, stream=None, probing_length=DEFAULT_PROBING_LENGTH,
+                   status_out=None) -> int:
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(num_per_key_out, 64, "num_per_key_out")
+        n = ctypes.c_uint64(0)
+        _check(self._fn("num_values")(self._h, _ptr(keys_in), keys_in.numel(), ctypes.byref(n),
+                                      _ptr(num_per_key_out), _stream(stream), probing_length, _ptr(status_out)),
+               "num_values")
+        return n.value
+
+    def retrieve_all_keys(self, keys_out, stream=None) -> int:
+        n = ctypes.c_uint64(0)
+        _check(self._fn("retrieve_all_keys")(self._h, _ptr(keys_out), ctypes.byref(n), _stream(stream)),
+               "retrieve_all_keys")
+        return n.value
+
+    def retrieve_all(self, keys_out, begin_offsets_out, end_offsets_out, values_out, stream=None):
+        nk, nv = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(self._fn("retrieve_all")(self._h, _ptr(keys_out), ctypes.byref(nk), _ptr(begin_offsets_out),
+                                        _ptr(end_offsets_out), _ptr(values_out), ctypes.byref(nv),
+                                        _stream(stream)), "retrieve_all")
+        return nk.value, nv.value
+
+    def num_keys(self, stream=None) -> int:
+        return self._fn("num_keys")(self._h, _stream(stream))
+
+    def size(self, stream=None) -> int:
+        return self._fn("size")(self._h, _stream(stream))
+
+    def load_factor(self, stream=None) -> float:
+        return self._fn("load_factor")(self._h, _stream(stream))
+
+    def storage_density(self, stream=None) -> float:
+        return self._fn("storage_density")(self._h, _stream(stream))
+
+    def capacity(self) -> int:
+        return self._fn("capacity")(self._h)
+
+    def bytes_total(self) -> int:
+        return self._fn("bytes_total")(self._h)
+
+    def max_values_per_key(self) -> int:
+        return self._fn("max_values_per_key")(self._h)
+
+    def is_initialized(self) -> bool:
+        return bool(self._fn("is_initialized")(self._h))
+
+    def peek_status(self, stream=None) -> Status:
+        return Status(self._fn("peek_status")(self._h, _stream(stream)))
+
+    def pop_status(self, stream=None) -> Status:
+        return Status(self._fn("pop_status")(self._h, _stream(stream)))
+
+    def cg_size(self) -> int:
+        return self._fn("cg_size")(self._h)
+
+
+class BucketListHashTable:
+    """BucketListHashTable<K, V> (include/warpcore/bucket_list_hash_table.cuh)."""
+
+    _NAMES = {(32, 32): "u32u32", (64, 64): "u64u64"}
+
+    def __init__(self, key_capacity: int, value_capacity: int, key_bits: int = 32, value_bits: int = 32,
+                 seed: int = DEFAULT_SEED, grow_factor: float = 1.1, min_bucket_size: int = 1,
+                 max_bucket_size: int = 1023, max_values_per_key: int = (1 << 20) - 1, no_init: bool = False,
+                 cg_size: int = 0):
+        self.key_bits, self.value_bits = key_bits, value_bits
+        self._p = f"wc_bucketlist_{self._NAMES[(key_bits, value_bits)]}_"
+        self._lib = _capi.lib()
+        self._h = self._fn("create")(key_capacity, value_capacity, seed, grow_factor, min_bucket_size,
+                                     max_bucket_size, max_values_per_key, int(no_init), cg_size)
+        if not self._h:
+            raise ValueError(f"unsupported configuration (cg_size={cg_size})")
+
+    def _fn(self, name):
+        return getattr(self._lib, self._p + name)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._fn("destroy")(h)
+
+    def init(self, seed: int | None = None, stream=None):
+        if seed is None:
+            _check(self._fn("init")(self._h, _stream(stream)), "init")
+        else:
+            _check(self._fn("init_seed")(self._h, seed, _stream(stream)), "init")
+
+    def insert(self, keys_in, values_in, stream=None, probing_length=DEFAULT_PROBING_LENGTH, status_out=None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(values_in, self.value_bits, "values_in")
+        _width(status_out, 32, "status_out")
+        _check(self._fn("insert")(self._h, _ptr(keys_in), _ptr(values_in), keys_in.numel(), _stream(stream),
+                                  probing_length, _ptr(status_out)), "insert")
+
+    def retrieve(self, keys_in, begin_offsets_out, end_offsets_out, values_out, stream=None,
+                 probing_length=DEFAULT_PROBING_LENGTH, status_out=None) -> int:
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(values_out, self.value_bits, "values_out")
+        n = ctypes.c_uint64(0)
+        _check(self._fn("retrieve")(self._h, _ptr(keys_in), keys_in.numel(), _ptr(begin_offsets_out),
+                                    _ptr(end_offsets_out), _ptr(values_out), ctypes.byref(n), _stream(stream),
+                                    probing_length, _ptr(status_out)), "retrieve")
+        return n.value
+
+    def num_values(self, keys_in=None, num_per_key_out=None, stream=None, probing_length=DEFAULT_PROBING_LENGTH,
+                   status_out=None) -> int:
+        """Values of keys_in (with per-key counts), or of the whole table when keys_in is None."""
+        if keys_in is None:
+            return self._fn("total_values")(self._h, _stream(stream))
+        _width(keys_in, self.key_bits, "keys_in")
+        n = ctypes.c_uint64(0)
+        _check(self._fn("num_values")(self._h, _ptr(keys_in), keys_in.numel(), ctypes.byref(n),
+                                      _ptr(num_per_key_out), _stream(stream), probing_length, _ptr(status_out)),
+               "num_values")
+        return n.value
+
+    def retrieve_all_keys(self, keys_out, stream=None) -> int:
+        n = ctypes.c_uint64(0)
+        _check(self._fn("retrieve_all_keys")(self._h, _ptr(keys_out), ctypes.byref(n), _stream(stream)),
+               "retrieve_all_keys")
+        return n.value
+
+    def retrieve_all(self, keys_out, begin_offsets_out, end_offsets_out, values_out, stream=None):
+        nk, nv = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        _check(self._fn("retrieve_all")(self._h, _ptr(keys_out), ctypes.byref(nk), _ptr(begin_offsets_out),
+                                        _ptr(end_offsets_out), _ptr(values_out), ctypes.byref(nv),
+                                        _stream(stream)), "retrieve_all")
+        return nk.value, nv.value
+
+    def num_keys(self, stream=None) -> int:
+        return self._fn("num_keys")(self._h, _stream(stream))
+
+    def key_load_factor(self, stream=None) -> float:
+        return self._fn("key_load_factor")(self._h, _stream(stream))
+
+    def value_load_factor(self, stream=None) -> float:
+        return self._fn("value_load_factor")(self._h, _stream(stream))
+
+    def storage_density(self, stream=None) -> float:
+        return self._fn("storage_density")(self._h, _stream(stream))
+
+    def relative_storage_density(self, stream=None) -> float:
+        return self._fn("relative_storage_density")(self._h, _stream(stream))
+
+    def key_capacity(self) -> int:
+        return self._fn("key_capacity")(self._h)
+
+    def value_capacity(self) -> int:
+        return self._fn("value_capacity")(self._h)
+
+    def value_cells_used(self, stream=None) -> int:
+        return self._fn("value_cells_used")(self._h, _stream(stream))
+
+    def bytes_total(self) -> int:
+        return self._fn("bytes_total")(self._h)
+
+    def max_values_per_key(self) -> int:
+        return self._fn("max_values_per_key")(self._h)
+
+    def is_initialized(self) -> bool:
+        return bool(self._fn("is_initialized")(self._h))
+
+    def peek_status(self, stream=None) -> Status:
+        return Status(self._fn("peek_status")(self._h, _stream(stream)))
+
+    def pop_status(self, stream=None) -> Status:
+        return Status(self._fn("pop_status")(self._h, _stream(stream)))
+
+    def cg_size(self) -> int:
+        return self._fn("cg_size")(self._h)
