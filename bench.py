@@ -182,6 +182,130 @@ def run_counting(g: Gpu, log2n: int, log2_distinct: int, s: float, steps: int, w
     return res
 
 
+def run_batched(g: Gpu, num_tables: int = 65536, steps: int = 3, warmup: int = 1, tune: bool = True,
+                sample_tables: int = 256, seed: int = 0xC7):
+    """65,536 SingleValueHashTable<u32,u32> of capacity log-uniform in
+    2^6..2^16 slots, each filled to a load factor uniform in [0.5, 0.9]:
+    insert of every table's keys, then retrieve of them, one persistent
+    launch each. Beside it, plain per-table launches on a sample."""
+    import numpy as np
+    torch = g.torch
+    from warpcore_amd.tables import SingleValueHashTable, TableBatch
+    rng = np.random.default_rng(seed)
+    caps = np.exp2(rng.uniform(6, 16, num_tables)).astype(np.uint64)
+    n_t = np.maximum((caps * rng.uniform(0.5, 0.9, num_tables)).astype(np.uint64), 1)
+    off = np.concatenate([[0], np.cumsum(n_t)]).astype(np.uint64)
+    n = int(off[-1])
+    keys, values, out = g.empty(n, 32), g.empty(n, 32), g.empty(n, 32)
+    with torch.cuda.stream(g.stream):
+        g.check(g.lib.wc_gen_unique_u32(g.p(keys), n, 0x7000, g.sp), "gen keys")
+        g.check(g.lib.wc_gen_uniform_u32(g.p(values), n, seed, g.sp), "gen values")
+    offs = torch.from_numpy(off.view(np.int64)).to("cuda")
+    batch = TableBatch(caps, 32, 32, cg_size=16)
+    g.stream.synchronize()
+    launch, tuned_ms = None, None
+    if tune:
+        launch, tuned_ms = batch.tune(keys, values, offs, keys, offs, out, stream=g.stream)
+    for _ in range(warmup):
+        batch.init(stream=g.stream)
+        batch.insert(keys, values, offs, stream=g.stream)
+        batch.retrieve(keys, offs, out, stream=g.stream)
+    g.stream.synchronize()
+    ev = [[g.event() for _ in range(3)] for _ in range(steps)]
+    for i in range(steps):
+        batch.init(stream=g.stream)
+        ev[i][0].record(g.stream)
+        batch.insert(keys, values, offs, stream=g.stream)
+        ev[i][1].record(g.stream)
+        batch.retrieve(keys, offs, out, stream=g.stream)
+        ev[i][2].record(g.stream)
+    g.stream.synchronize()
+    ins = min(e[0].elapsed_time(e[1]) for e in ev)
+    ret = min(e[1].elapsed_time(e[2]) for e in ev)
+    ok = bool(torch.equal(out, values))
+    # plain launches: one SingleValueHashTable per table for a sample of tables
+    pick = np.linspace(0, num_tables - 1, sample_tables).astype(np.int64)
+    tables = [SingleValueHashTable(int(caps[t]), 32, 32, cg_size=16) for t in pick]
+    g.stream.synchronize()
+    e0, e1 = g.event(), g.event()
+    e0.record(g.stream)
+    sample_keys = 0
+    for t, tab in zip(pick, tables):
+        lo, hi = int(off[t]), int(off[t + 1])
+        tab.insert(keys[lo:hi], values[lo:hi], stream=g.stream)
+        tab.retrieve(keys[lo:hi], out[lo:hi], stream=g.stream)
+        sample_keys += hi - lo
+    e1.record(g.stream)
+    g.stream.synchronize()
+    plain_ms = e0.elapsed_time(e1)
+    res = {"tables": num_tables, "keys": n, "total_slots": batch.total_slots(),
+           "gops": round(2 * n / (ins + ret) / 1e6, 3), "insert_gops": round(n / ins / 1e6, 3),
+           "retrieve_gops": round(n / ret / 1e6, 3), "launch": list(launch) if launch else None,
+           "results_ok": ok, "plain_sample_tables": sample_tables,
+           "plain_per_table_gops": round(2 * sample_keys / plain_ms / 1e6, 3)}
+    del batch, tables, keys, values, out, offs
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_serving(g: Gpu, num_tables: int = 64, total_bytes: float = 100e9, num_requests: int = 2000,
+                load: float = 0.5, seed: int = 0xC8):
+    """Open-loop request stream against 64 resident SingleValueHashTable
+    <u64,u64> (about 100 GB of slots): sizes log-uniform 2^8..2^22 keys,
+    30 % insert / 60 % retrieve / 10 % erase, uniform u64 keys with 20 %
+    Zipf(1.0) hot keys. Phase 1: every request at t=0 (throughput). Phase 2:
+    Poisson arrivals at `load` x the one-launch-per-request throughput
+    (p50/p99 latency). Both through the server and one launch per request."""
+    import numpy as np
+    torch = g.torch
+    from warpcore_amd.tables import Server
+    rng = np.random.default_rng(seed)
+    caps = np.full(num_tables, int(total_bytes / 16 / num_tables), np.uint64)
+    sizes = np.exp2(rng.uniform(8, 22, num_requests)).astype(np.uint64)
+    ops = rng.choice([0, 1, 2], size=num_requests, p=[0.3, 0.6, 0.1]).astype(np.uint32)
+    tabs = rng.integers(0, num_tables, size=num_requests).astype(np.uint32)
+    n = int(sizes.sum())
+    keys, values, out, st = g.empty(n, 64), g.empty(n, 64), g.empty(n, 64), g.empty(n, 32)
+    hot = g.empty(n, 64)
+    with torch.cuda.stream(g.stream):
+        g.check(g.lib.wc_gen_uniform_u64(g.p(keys), n, seed, g.sp), "gen keys")
+        g.check(g.lib.wc_gen_zipf_u64(g.p(hot), n, 1 << 20, 1.0, seed + 1, 0x9000, g.sp), "gen hot")
+        g.check(g.lib.wc_gen_uniform_u64(g.p(values), n, seed + 2, g.sp), "gen values")
+        take = torch.rand(n, device="cuda") < 0.2
+        keys[take] = hot[take]
+        # 0 and ~0 are not keys
+        keys[(keys == 0) | (keys == -1)] = 0x1234567
+    g.stream.synchronize()
+    del hot, take
+    res = {"tables": num_tables, "table_slots": int(caps[0]), "requests": num_requests, "keys": n}
+
+    def one(mode, arrival):
+        srv = Server(caps, 64, 64)
+        lat, mk = srv.drive(mode, ops, tabs, sizes, arrival, keys, values, out, st)
+        steps = srv.stats()["steps"] if mode == 0 else num_requests
+        del srv
+        torch.cuda.empty_cache()
+        return lat, mk, steps
+
+    zero = np.zeros(num_requests)
+    lat_b, mk_b, _ = one(1, zero)
+    lat_s, mk_s, steps_s = one(0, zero)
+    res["throughput"] = {"server_gkeys_s": round(n / mk_s / 1e3, 3), "launch_per_request_gkeys_s":
+                         round(n / mk_b / 1e3, 3), "speedup": round(mk_b / mk_s, 3), "server_steps": steps_s}
+    rate = load * num_requests / mk_b          # requests per microsecond
+    arrival = np.cumsum(rng.exponential(1.0 / rate, size=num_requests))
+    lat_b, mk_b2, _ = one(1, arrival)
+    lat_s, mk_s2, steps_s2 = one(0, arrival)
+    pct = lambda a, q: round(float(np.percentile(a, q)), 1)
+    res["latency"] = {"offered_load": load, "offered_gkeys_s": round(n / arrival[-1] / 1e3, 3),
+                      "server_p50_us": pct(lat_s, 50), "server_p99_us": pct(lat_s, 99),
+                      "launch_per_request_p50_us": pct(lat_b, 50), "launch_per_request_p99_us": pct(lat_b, 99),
+                      "server_steps": steps_s2}
+    del keys, values, out, st
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline(log2n: int = 24, reps: int = 3):
     """OpenMP table with the same hashing, probing and CAS protocol, on a
     2^log2n-key sample of the same workload (u64/u64, LF 0.9, 50 % hits)."""
@@ -217,7 +341,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2-keys", type=int, default=28)
-    ap.add_argument("--no-extras", action="store_true", help="skip the u32 and counting workloads")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the u32, counting, batched-table and serving workloads")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched small tables")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving loop")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-log2-keys", type=int, default=24)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -250,6 +377,14 @@ def main():
                                      "occurrences": rc["n"], "distinct": rc["distinct"],
                                      "capacity": rc["capacity"]},
         }
+        if not args.no_batched:
+            log("[bench] batched tables ...")
+            extras["batched_65536_u32u32"] = run_batched(g)
+            log(f"[bench] batched: {extras['batched_65536_u32u32']}")
+        if not args.no_serving:
+            log("[bench] serving ...")
+            extras["serving_64x1.56GB_u64u64"] = run_serving(g)
+            log(f"[bench] serving: {extras['serving_64x1.56GB_u64u64']}")
     if args.calibrate_copy_mib:
         torch = g.torch
         a = torch.empty(args.calibrate_copy_mib << 20, dtype=torch.uint8, device="cuda")

@@ -433,6 +433,109 @@ WC_API uint32_t wc_bucketlist_u64u64_pop_status(void* table, void* stream);
 WC_API uint32_t wc_bucketlist_u64u64_cg_size(void* table);
 
 /* ------------------------------------------------------------------------
+ * TableBatch<K, V, CG>: many small SingleValueHashTables in one arena (no
+ * reference counterpart; the reference builds one table per object with at
+ * least 131101*cg slots). Table t gets the smallest prime >= SDIV(min, cg)
+ * windows of cg slots (cg 4, 8 or 16; 0 = 16).
+ *   create       min_capacities is a HOST array of num_tables slot counts
+ *   insert / retrieve  keys of all tables concatenated; offsets is a DEVICE
+ *                array of num_tables + 1 (table t owns [offsets[t],
+ *                offsets[t+1])); one persistent work-stealing launch; per
+ *                key results and statuses as SingleValueHashTable's
+ *   set_launch / get_launch  keys per lockstep batch (4 or 8), keys per
+ *                work item, blocks per CU; tune times the candidates on the
+ *                given insert + retrieve workload, keeps the fastest and
+ *                leaves the batch empty (launch_out: 3 values, ms_out: best
+ *                insert + retrieve time)
+ *   copy_status_words  per-table status words into a device buffer
+ * ---------------------------------------------------------------------- */
+/* TableBatch<uint32_t, uint32_t> */
+WC_API void* wc_batch_u32u32_create(const uint64_t* min_capacities, uint64_t num_tables, uint32_t seed, uint32_t cg_size);
+WC_API void wc_batch_u32u32_destroy(void* batch);
+WC_API int wc_batch_u32u32_init(void* batch, void* stream);
+WC_API int wc_batch_u32u32_insert(void* batch, const uint32_t* keys_in, const uint32_t* values_in, const uint64_t* offsets, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_batch_u32u32_retrieve(void* batch, const uint32_t* keys_in, const uint64_t* offsets, uint64_t num_in, uint32_t* values_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_batch_u32u32_set_launch(void* batch, uint32_t keys_per_batch, uint64_t item_keys, uint32_t blocks_per_cu);
+WC_API int wc_batch_u32u32_get_launch(void* batch, uint64_t* launch_out);
+WC_API int wc_batch_u32u32_tune(void* batch, const uint32_t* keys_in, const uint32_t* values_in, const uint64_t* offsets, uint64_t num_in, const uint32_t* queries, const uint64_t* query_offsets, uint64_t num_queries, uint32_t* values_out, void* stream, uint64_t* launch_out, double* ms_out);
+WC_API uint64_t wc_batch_u32u32_num_tables(void* batch);
+WC_API uint64_t wc_batch_u32u32_capacity(void* batch, uint64_t table);
+WC_API uint64_t wc_batch_u32u32_total_slots(void* batch);
+WC_API uint64_t wc_batch_u32u32_bytes_total(void* batch);
+WC_API int wc_batch_u32u32_copy_status_words(void* batch, uint32_t* words_out, void* stream);
+WC_API uint32_t wc_batch_u32u32_cg_size(void* batch);
+
+/* TableBatch<uint64_t, uint64_t> */
+WC_API void* wc_batch_u64u64_create(const uint64_t* min_capacities, uint64_t num_tables, uint64_t seed, uint32_t cg_size);
+WC_API void wc_batch_u64u64_destroy(void* batch);
+WC_API int wc_batch_u64u64_init(void* batch, void* stream);
+WC_API int wc_batch_u64u64_insert(void* batch, const uint64_t* keys_in, const uint64_t* values_in, const uint64_t* offsets, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_batch_u64u64_retrieve(void* batch, const uint64_t* keys_in, const uint64_t* offsets, uint64_t num_in, uint64_t* values_out, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API int wc_batch_u64u64_set_launch(void* batch, uint32_t keys_per_batch, uint64_t item_keys, uint32_t blocks_per_cu);
+WC_API int wc_batch_u64u64_get_launch(void* batch, uint64_t* launch_out);
+WC_API int wc_batch_u64u64_tune(void* batch, const uint64_t* keys_in, const uint64_t* values_in, const uint64_t* offsets, uint64_t num_in, const uint64_t* queries, const uint64_t* query_offsets, uint64_t num_queries, uint64_t* values_out, void* stream, uint64_t* launch_out, double* ms_out);
+WC_API uint64_t wc_batch_u64u64_num_tables(void* batch);
+WC_API uint64_t wc_batch_u64u64_capacity(void* batch, uint64_t table);
+WC_API uint64_t wc_batch_u64u64_total_slots(void* batch);
+WC_API uint64_t wc_batch_u64u64_bytes_total(void* batch);
+WC_API int wc_batch_u64u64_copy_status_words(void* batch, uint32_t* words_out, void* stream);
+WC_API uint32_t wc_batch_u64u64_cg_size(void* batch);
+
+/* ------------------------------------------------------------------------
+ * Serving loop (no reference counterpart): device-resident
+ * SingleValueHashTables (u64u64: cg 8, u32u32: cg 16) serve insert (op 0),
+ * retrieve (op 1) and erase (op 2) requests from many host threads. A
+ * scheduler thread coalesces admitted requests into steps: one persistent
+ * launch over all requests of a step, captured as a HIP graph when
+ * use_graph. Per table, requests run in arrival order (a step holds a run
+ * of lookups or one insert/erase per table). An insert that does not fit
+ * its table's slot budget (max_load * capacity, never given back) is
+ * refused: every key reports out_of_memory and wait returns 0.
+ *   create       table_capacities: HOST array (minimum capacities);
+ *                max_requests per step (0 = 1024), item_keys per work item
+ *                (0 = 1024), max_load (<= 0: 0.9), blocks_per_cu (0 = 8)
+ *   submit       device buffers, kept valid until the request is done;
+ *                returns the request id (-1 on a bad argument)
+ *   wait / drain block until one / every request is done
+ *   latency_us   arrival to completion of a done request
+ *   step_of      the step a request ran in (1, 2, ...)
+ *   stats        out3 = {requests done, steps, keys}
+ *   drive        replays num_requests requests (keys/values/outputs are
+ *                slices of concatenated device arrays, sizes[i] each) at
+ *                arrival_us[i] after the call starts; mode 0 through the
+ *                server, mode 1 one plain table call per request in arrival
+ *                order on the server's stream; fills latency_us_out[i] and
+ *                *makespan_us
+ * ---------------------------------------------------------------------- */
+/* Server over SingleValueHashTable<uint64_t, uint64_t> */
+WC_API void* wc_server_u64u64_create(const uint64_t* table_capacities, uint64_t num_tables, uint64_t max_requests, uint64_t item_keys, float max_load, int use_graph, int blocks_per_cu);
+WC_API void wc_server_u64u64_destroy(void* server);
+WC_API int64_t wc_server_u64u64_submit(void* server, uint32_t op, uint32_t table, const uint64_t* keys_in, const uint64_t* values_in, uint64_t* values_out, uint32_t* status_out, uint64_t num_in);
+WC_API int wc_server_u64u64_wait(void* server, int64_t id);
+WC_API int wc_server_u64u64_drain(void* server);
+WC_API double wc_server_u64u64_latency_us(void* server, int64_t id);
+WC_API uint64_t wc_server_u64u64_step_of(void* server, int64_t id);
+WC_API int wc_server_u64u64_stats(void* server, uint64_t* out3);
+WC_API uint64_t wc_server_u64u64_table_size(void* server, uint32_t table);
+WC_API uint64_t wc_server_u64u64_table_capacity(void* server, uint32_t table);
+WC_API uint32_t wc_server_u64u64_table_status(void* server, uint32_t table);
+WC_API int wc_server_u64u64_drive(void* server, int mode, const uint32_t* ops, const uint32_t* tables, const uint64_t* sizes, const double* arrival_us, uint64_t num_requests, const uint64_t* keys_in, const uint64_t* values_in, uint64_t* values_out, uint32_t* status_out, double* latency_us_out, double* makespan_us);
+
+/* Server over SingleValueHashTable<uint32_t, uint32_t> */
+WC_API void* wc_server_u32u32_create(const uint64_t* table_capacities, uint64_t num_tables, uint64_t max_requests, uint64_t item_keys, float max_load, int use_graph, int blocks_per_cu);
+WC_API void wc_server_u32u32_destroy(void* server);
+WC_API int64_t wc_server_u32u32_submit(void* server, uint32_t op, uint32_t table, const uint32_t* keys_in, const uint32_t* values_in, uint32_t* values_out, uint32_t* status_out, uint64_t num_in);
+WC_API int wc_server_u32u32_wait(void* server, int64_t id);
+WC_API int wc_server_u32u32_drain(void* server);
+WC_API double wc_server_u32u32_latency_us(void* server, int64_t id);
+WC_API uint64_t wc_server_u32u32_step_of(void* server, int64_t id);
+WC_API int wc_server_u32u32_stats(void* server, uint64_t* out3);
+WC_API uint64_t wc_server_u32u32_table_size(void* server, uint32_t table);
+WC_API uint64_t wc_server_u32u32_table_capacity(void* server, uint32_t table);
+WC_API uint32_t wc_server_u32u32_table_status(void* server, uint32_t table);
+WC_API int wc_server_u32u32_drive(void* server, int mode, const uint32_t* ops, const uint32_t* tables, const uint64_t* sizes, const double* arrival_us, uint64_t num_requests, const uint32_t* keys_in, const uint32_t* values_in, uint32_t* values_out, uint32_t* status_out, double* latency_us_out, double* makespan_us);
+
+/* ------------------------------------------------------------------------
  * Synthetic inputs (the role of include/warpcore/random_distributions.cuh:
  * uniform 50-135, unique 144-246, zipf 259-326). Counter-based: element i
  * depends only on (seed or offset, i).
@@ -462,5 +565,13 @@ WC_API uint32_t wc_unique_key_u32(uint64_t offset, uint64_t i);
  * tables by cooperative groups of cg_size lanes per key. */
 WC_API void wc_set_lane_table_bytes(uint64_t bytes);
 WC_API uint64_t wc_get_lane_table_bytes(void);
+
+/* Bulk kernels of tables probed by cooperative groups (no reference
+ * counterpart), a bit mask: bit 0 = lockstep inserts, bit 1 = lockstep
+ * lookups; a clear bit selects the streaming per-key state machine. Default
+ * 1 (lockstep inserts, streaming lookups). All choices give identical
+ * results. */
+WC_API void wc_set_group_kernel(int kernel);
+WC_API int wc_get_group_kernel(void);
 
 #endif /* WARPCORE_C_H */

@@ -79,19 +79,23 @@ def dev_empty(n: int, bits: int, fill: int = 0):
 
 class kernel_shape:
     """Context manager selecting the bulk-kernel shape: 'lanes' (one lane per
-    key) or 'groups' (cooperative groups), by moving the table-size limit of
-    the lane-per-key path."""
+    key), 'groups' (cooperative groups, streaming lookups) or 'lockstep'
+    (cooperative groups, lockstep lookups), by moving the table-size limit
+    of the lane-per-key path and choosing the group lookup kernel."""
 
     def __init__(self, shape: str):
         self.limit = (1 << 62) if shape == "lanes" else 0
+        self.group_kernel = 0 if shape == "groups" else 3   # all streaming / all lockstep
 
     def __enter__(self):
         from warpcore_amd import _capi
         self.lib = _capi.lib()
-        self.saved = self.lib.wc_get_lane_table_bytes()
+        self.saved = self.lib.wc_get_lane_table_bytes(), self.lib.wc_get_group_kernel()
         self.lib.wc_set_lane_table_bytes(self.limit)
+        self.lib.wc_set_group_kernel(self.group_kernel)
         return self
 
     def __exit__(self, *exc):
-        self.lib.wc_set_lane_table_bytes(self.saved)
+        self.lib.wc_set_lane_table_bytes(self.saved[0])
+        self.lib.wc_set_group_kernel(self.saved[1])
         return False

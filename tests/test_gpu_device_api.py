@@ -34,8 +34,24 @@ def dlib(cuda):
     return lib
 
 
+_keep = []
+
+
 def _p(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    """Device pointer of t; t is kept alive until the test module ends (a
+    temporary tensor freed before the call could be handed to the next
+    allocation)."""
+    if t is None:
+        return None
+    _keep.append(t)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+@pytest.fixture(autouse=True)
+def _release():
+    yield
+    torch.cuda.synchronize()
+    _keep.clear()
 
 
 def test_filtered_histogram_from_device(dlib):

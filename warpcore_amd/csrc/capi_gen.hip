@@ -80,3 +80,8 @@ extern "C" uint32_t wc_unique_key_u32(uint64_t offset, uint64_t i) { return rand
 // one lane per key, larger ones by cooperative groups.
 extern "C" void wc_set_lane_table_bytes(uint64_t bytes) { detail::set_lane_table_bytes(bytes); }
 extern "C" uint64_t wc_get_lane_table_bytes(void) { return detail::lane_table_bytes(); }
+
+// Kernel used for bulk lookups probed by cooperative groups: 0 = streaming
+// state machine, 1 = lockstep batches.
+extern "C" void wc_set_group_kernel(int kernel) { detail::set_group_kernel(kernel); }
+extern "C" int wc_get_group_kernel(void) { return detail::group_kernel(); }

@@ -1,0 +1,493 @@
+// Serving loop: device-resident SingleValueHashTables serve insert /
+// retrieve / erase requests from many clients through continuous batching.
+//
+// No reference counterpart (the reference is a library of one-call-per-
+// batch operations). Clients submit requests (device buffers of keys,
+// values, outputs); a scheduler thread admits them, coalesces every
+// admitted request into one step, and runs a step as one fixed-shape
+// sequence of launches (descriptor upload, work-item listing, one
+// persistent kernel over (request, chunk) items), captured once as a HIP
+// graph and replayed. Steps are issued back to back on one stream while
+// the host already builds the next one, so requests that arrive while a
+// step runs join the following step; a request completes when the event
+// recorded after its step has fired.
+//
+// Admission against the HBM budget: every table has a slot budget of
+// max_load * capacity. An insert of n keys reserves n slots of its table
+// (a key may be a duplicate, so this is an upper bound); slots are never
+// given back, because erase leaves tombstones that keep their slots, so an
+// insert that does not fit its table's remaining budget is refused (status
+// out_of_memory for every key). A step holds at most max_requests requests
+// and max_items work items; what does not fit waits for the next step.
+//
+// Order: per table, requests run in arrival order. A step takes, per
+// table, either any run of consecutive lookups, or one insert or erase, so
+// no two requests of one step conflict; steps run in order on the stream.
+// Each request therefore sees exactly the table state it would see if all
+// requests of its table ran one at a time in arrival order, which is what
+// the tests replay.
+#pragma once
+
+#include "base.hpp"
+#include "engine.hpp"
+#include "lockstep_probing.hpp"
+#include "scan.hpp"
+#include "single_value_hash_table.hpp"
+#include "status.hpp"
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+namespace warpcore {
+namespace serving {
+
+enum class Op : std::uint32_t { insert = 0, retrieve = 1, erase = 2 };
+
+// Per-request descriptor read by the step kernel.
+template<class K, class V>
+struct RequestDesc {
+    const K* keys;
+    const V* values;        // insert
+    V* values_out;          // retrieve
+    std::uint32_t* status_out;   // may be null
+    index_t n;
+    std::uint32_t table;
+    std::uint32_t op;
+};
+
+}  // namespace serving
+
+namespace kernels {
+
+template<class Desc>
+__global__ __launch_bounds__(256) void serve_count_items(const Desc* __restrict__ reqs,
+                                                         const std::uint32_t* __restrict__ num_reqs,
+                                                         index_t max_reqs, index_t item_keys,
+                                                         index_t* __restrict__ items) {
+    const index_t r = index_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r < max_reqs) items[r] = r < *num_reqs ? SDIV(reqs[r].n, item_keys) : 0;
+}
+
+template<class T>
+__global__ __launch_bounds__(256) void serve_fill_items(const index_t* __restrict__ first,
+                                                        const index_t* __restrict__ items, index_t max_reqs,
+                                                        std::uint32_t* __restrict__ item_req,
+                                                        index_t* __restrict__ total, index_t* __restrict__ counter) {
+    const index_t r = index_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r < max_reqs)
+        for (index_t j = 0; j < items[r]; ++j) item_req[first[r] + j] = std::uint32_t(r);
+    if (r == 0) {
+        *total = first[max_reqs - 1] + items[max_reqs - 1];
+        *counter = 0;
+    }
+}
+
+// Erase keys_in[first], keys_in[first + 1], ... below end, one key per
+// group at a time (single_value_hash_table.cuh:425-468).
+template<class View, class StatusHandler>
+WC_D void serve_erase_range(const typename View::key_type* keys_in, index_t first, index_t end, const View& view,
+                            index_t probing_length, typename StatusHandler::base_type* status_out,
+                            detail::DeferredJoin& join) {
+    constexpr unsigned CG = unsigned(View::cg_size());
+    const tile<CG> group;
+    for (index_t i = first; i < end; ++i) {
+        const Status st = view.erase(keys_in[i], group, probing_length, join);
+        if (group.thread_rank() == 0) StatusHandler::handle(st, status_out, i);
+    }
+}
+
+template<class View, int UR, int UI>
+__global__ __launch_bounds__(256) void serve_step(const serving::RequestDesc<typename View::key_type,
+                                                                             typename View::value_type>* __restrict__ reqs,
+                                                  const std::uint32_t* __restrict__ item_req,
+                                                  const index_t* __restrict__ first_item,
+                                                  const index_t* __restrict__ total_items,
+                                                  index_t* __restrict__ counter, index_t item_keys,
+                                                  const View* __restrict__ views, index_t probing_length) {
+    using K = typename View::key_type;
+    using W = std::uint32_t;
+    constexpr unsigned CG = unsigned(View::cg_size());
+    constexpr int UMAX = UR > UI ? UR : UI;
+    __shared__ detail::StagedKey<K, W> stage[256 / CG * UMAX];
+    auto* my = stage + (threadIdx.x / CG) * UMAX;
+    const unsigned rank = lane_id() & (CG - 1);
+    const index_t total = *total_items;
+    using H = status_handlers::ReturnStatus;
+    while (true) {
+        index_t item = 0;
+        if (rank == 0) item = detail::atomic_add(counter, index_t(1));
+        item = tile<CG>::shfl(item, 0);
+        if (item >= total) break;
+        const std::uint32_t r = item_req[item];
+        const auto d = reqs[r];
+        const index_t begin = (item - first_item[r]) * item_keys;
+        const index_t end = begin + item_keys < d.n ? begin + item_keys : d.n;
+        const View view = views[d.table];
+        const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, view.geo.windows);
+        detail::DeferredJoin join;
+        auto* st = reinterpret_cast<Status*>(d.status_out);
+        if (d.op == std::uint32_t(serving::Op::insert))
+            detail::insert_lockstep_range<View, H, UI, W, true>(d.keys, d.values, begin, end, UI, view, budget, st,
+                                                                my, join);
+        else if (d.op == std::uint32_t(serving::Op::retrieve))
+            detail::retrieve_lockstep_range<View, H, UR, W>(d.keys, begin, end, UR, d.values_out, view, budget, st,
+                                                            my, join);
+        else
+            serve_erase_range<View, H>(d.keys, begin, end, view, probing_length, st, join);
+        const status_base_t all = tile<CG>::reduce_or(join.acc);
+        if (rank == 0 && all != 0) view.status->atomic_join(Status::from_base(all));
+    }
+}
+
+}  // namespace kernels
+
+namespace serving {
+
+struct Stats {
+    index_t requests = 0, steps = 0, keys = 0;
+};
+
+// K/V: u32 or u64; CG: probing group (8 for 16-byte slots, 16 for 8-byte).
+template<class K, class V, index_t CG>
+class Server {
+public:
+    using table_type = SingleValueHashTable<K, V, defaults::empty_key<K>(), defaults::tombstone_key<K>(),
+                                            defaults::probing_scheme_t<K, CG>>;
+    using view_type = typename table_type::view_type;
+    using desc_type = RequestDesc<K, V>;
+    using clock = std::chrono::steady_clock;
+
+    struct Config {
+        index_t max_requests = 1024;      // requests per step (fixed shape)
+        index_t item_keys = 1024;         // keys per work item
+        float max_load = 0.9f;            // admission: slot budget per table
+        int blocks_per_cu = 8;            // persistent grid
+        bool use_graph = true;            // replay a captured step
+    };
+
+    struct Request {
+        desc_type d;
+        index_t id;
+        clock::time_point arrival, done;
+        std::uint64_t step = 0;          // step that ran it (0 = not yet)
+        std::atomic<int> state{0};       // 0 queued, 1 running, 2 done
+        bool refused = false;
+    };
+
+    Server(const index_t* table_capacities, index_t num_tables, Config cfg = Config{}) : cfg_(cfg) {
+        for (index_t t = 0; t < num_tables; ++t) {
+            tables_.emplace_back(new table_type(table_capacities[t]));
+            // the step kernel keeps window indices in 32 bits
+            if (!tables_.back()->is_initialized() || tables_.back()->capacity() / CG >= (index_t(1) << 31)) {
+                ok_ = false;
+                return;
+            }
+            budget_.push_back(index_t(double(tables_.back()->capacity()) * cfg_.max_load));
+            reserved_.push_back(0);
+        }
+        std::vector<view_type> views;
+        for (auto& t : tables_) views.push_back(t->view());
+        ok_ = alloc(&views_d_, views.size()) && alloc(&item_req_, cfg_.max_requests * 64 + 1024) &&
+              alloc(&items_, cfg_.max_requests) && alloc(&first_, cfg_.max_requests) && alloc(&total_, 1) &&
+              alloc(&counter_, 1) && alloc(&parts_, detail::kScanParts) && alloc(&num_reqs_d_, 1);
+        for (int b = 0; b < 2 && ok_; ++b) {
+            ok_ = alloc(&desc_d_[b], cfg_.max_requests) &&
+                  hipHostMalloc(reinterpret_cast<void**>(&desc_h_[b]), sizeof(desc_type) * cfg_.max_requests) ==
+                      hipSuccess &&
+                  hipHostMalloc(reinterpret_cast<void**>(&num_reqs_h_[b]), sizeof(std::uint32_t)) == hipSuccess;
+        }
+        if (!ok_) return;
+        // blocking streams: work the client queued on the default stream
+        // (filling its buffers) is ordered before the steps
+        ok_ = hipStreamCreate(&stream_) == hipSuccess && hipStreamCreate(&side_) == hipSuccess &&
+              hipMemcpy(views_d_, views.data(), sizeof(view_type) * views.size(), hipMemcpyHostToDevice) == hipSuccess;
+        if (!ok_) return;
+        worker_ = std::thread([this] { loop(); });
+    }
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (worker_.joinable()) worker_.join();
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        for (auto& g : graphs_) if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        for (void* p : {(void*)views_d_, (void*)item_req_, (void*)items_, (void*)first_, (void*)total_,
+                        (void*)counter_, (void*)parts_, (void*)num_reqs_d_, (void*)desc_d_[0], (void*)desc_d_[1]})
+            if (p) (void)hipFree(p);
+        for (int b = 0; b < 2; ++b) {
+            if (desc_h_[b]) (void)hipHostFree(desc_h_[b]);
+            if (num_reqs_h_[b]) (void)hipHostFree(num_reqs_h_[b]);
+        }
+        if (stream_) (void)hipStreamDestroy(stream_);
+        if (side_) (void)hipStreamDestroy(side_);
+    }
+
+    bool ok() const noexcept { return ok_; }
+    table_type& table(index_t t) { return *tables_[t]; }
+    index_t num_tables() const noexcept { return tables_.size(); }
+    hipStream_t stream() const noexcept { return stream_; }
+
+    // Submit a request; the buffers must stay valid until it is done.
+    // Returns the request id. A request naming no table, or missing the
+    // keys, the values of an insert or the output of a lookup, is refused.
+    index_t submit(Op op, std::uint32_t table, const K* keys, const V* values, V* values_out,
+                   std::uint32_t* status_out, index_t n) {
+        auto r = std::make_shared<Request>();
+        r->d = desc_type{keys, values, values_out, status_out, n, table, std::uint32_t(op)};
+        r->arrival = clock::now();
+        const bool bad = table >= tables_.size() || (n != 0 && keys == nullptr) ||
+                         (op == Op::insert && n != 0 && values == nullptr) ||
+                         (op == Op::retrieve && n != 0 && values_out == nullptr);
+        index_t id;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            id = next_id_++;
+            r->id = id;
+            all_.push_back(r);
+            if (bad) {
+                finish_refused(*r);
+            } else {
+                pending_.push_back(r);
+            }
+        }
+        cv_.notify_all();
+        return id;
+    }
+
+    // Block until request `id` is done; returns true unless it was refused.
+    bool wait(index_t id) {
+        std::unique_lock<std::mutex> g(mu_);
+        done_cv_.wait(g, [&] { return all_[id]->state.load() == 2; });
+        return !all_[id]->refused;
+    }
+    void drain() {
+        std::unique_lock<std::mutex> g(mu_);
+        done_cv_.wait(g, [&] {
+            for (auto& r : all_) if (r->state.load() != 2) return false;
+            return true;
+        });
+    }
+    // latency of request id in microseconds (arrival to completion seen)
+    double latency_us(index_t id) {
+        std::lock_guard<std::mutex> g(mu_);
+        const auto& r = *all_[id];
+        return std::chrono::duration<double, std::micro>(r.done - r.arrival).count();
+    }
+    std::uint64_t step_of(index_t id) {
+        std::lock_guard<std::mutex> g(mu_);
+        return all_[id]->step;
+    }
+    Stats stats() {
+        std::lock_guard<std::mutex> g(mu_);
+        return stats_;
+    }
+
+private:
+    template<class T>
+    static bool alloc(T** p, index_t n) {
+        *p = nullptr;
+        return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1)) == hipSuccess;
+    }
+
+    void finish_refused(Request& r) {
+        // every key of a refused request reports out_of_memory; on a side
+        // stream, as the step stream may be capturing a graph
+        r.refused = true;
+        if (r.d.status_out && r.d.n)
+            storage::kernels::fill<std::uint32_t>
+                <<<detail::grid_for(r.d.n, 1, 8), block_size(), 0, side_>>>(r.d.status_out, r.d.n,
+                                                                            Status::out_of_memory().base());
+        (void)hipStreamSynchronize(side_);
+        r.done = clock::now();
+        r.state.store(2);
+        done_cv_.notify_all();
+    }
+
+    // Requests of the next step, chosen under mu_: per table in arrival
+    // order, a run of lookups or one mutation; inserts must fit the budget.
+    void pick(std::vector<std::shared_ptr<Request>>& step) {
+        std::vector<int> mode(tables_.size(), -1);   // -1 untouched, 0 lookups, 1 mutation, 2 closed
+        std::deque<std::shared_ptr<Request>> keep;
+        index_t items = 0;
+        const index_t max_items = cfg_.max_requests * 64 + 1024;
+        while (!pending_.empty()) {
+            auto r = pending_.front();
+            pending_.pop_front();
+            const std::uint32_t t = r->d.table;
+            const Op op = Op(r->d.op);
+            const index_t it = SDIV(r->d.n, cfg_.item_keys);
+            bool take = step.size() < cfg_.max_requests && items + it <= max_items;
+            if (take) {
+                if (mode[t] == 2 || mode[t] == 1) take = false;
+                else if (op == Op::retrieve) take = true;
+                else take = mode[t] == -1;
+            }
+            if (take && op == Op::insert && reserved_[t] + r->d.n > budget_[t]) {
+                finish_refused(*r);
+                continue;
+            }
+            if (it > max_items) {   // larger than any step
+                finish_refused(*r);
+                continue;
+            }
+            if (take) {
+                if (op == Op::insert) reserved_[t] += r->d.n;
+                mode[t] = op == Op::retrieve ? 0 : 1;
+                items += it;
+                step.push_back(r);
+            } else {
+                mode[t] = 2;   // later requests of this table wait behind it
+                keep.push_back(r);
+            }
+        }
+        pending_.swap(keep);
+    }
+
+    struct Graph {
+        hipGraphExec_t exec = nullptr;
+    };
+
+    // Work-item listing plus the persistent kernel of one step, reading the
+    // descriptors at desc_d_[b] (fixed shape: max_requests descriptors).
+    void enqueue_step(int b) {
+        const index_t R = cfg_.max_requests;
+        const unsigned rg = unsigned(SDIV(R, index_t(256)));
+        kernels::serve_count_items<desc_type><<<rg, 256, 0, stream_>>>(desc_d_[b], num_reqs_d_, R, cfg_.item_keys, items_);
+        detail::device_scan(items_, first_, R, parts_, false, stream_);
+        kernels::serve_fill_items<index_t><<<rg, 256, 0, stream_>>>(first_, items_, R, item_req_, total_, counter_);
+        const unsigned grid = unsigned(detail::num_compute_units() * cfg_.blocks_per_cu);
+        kernels::serve_step<view_type, detail::lockstep_keys<unsigned(CG)>(),
+                            detail::lockstep_insert_keys<unsigned(CG)>()>
+            <<<grid, block_size(), 0, stream_>>>(desc_d_[b], item_req_, first_, total_, counter_, cfg_.item_keys,
+                                                 views_d_, defaults::probing_length());
+    }
+
+    void launch(int b) {
+        // the request count and descriptors travel with the step
+        (void)hipMemcpyAsync(num_reqs_d_, num_reqs_h_[b], sizeof(std::uint32_t), hipMemcpyHostToDevice, stream_);
+        (void)hipMemcpyAsync(desc_d_[b], desc_h_[b], sizeof(desc_type) * cfg_.max_requests, hipMemcpyHostToDevice,
+                             stream_);
+        if (!cfg_.use_graph) {
+            enqueue_step(b);
+            return;
+        }
+        if (!graphs_[b].exec) {
+            hipGraph_t g = nullptr;
+            bool ok = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
+                enqueue_step(b);
+                ok = hipStreamEndCapture(stream_, &g) == hipSuccess &&
+                     hipGraphInstantiate(&graphs_[b].exec, g, nullptr, nullptr, 0) == hipSuccess;
+            }
+            if (g) (void)hipGraphDestroy(g);
+            if (!ok) {
+                graphs_[b].exec = nullptr;
+                cfg_.use_graph = false;
+                enqueue_step(b);
+                return;
+            }
+        }
+        (void)hipGraphLaunch(graphs_[b].exec, stream_);
+    }
+
+    void retire(std::deque<std::pair<hipEvent_t, std::vector<std::shared_ptr<Request>>>>& inflight) {
+        const auto now = clock::now();
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            for (auto& r : inflight.front().second) {
+                r->done = now;
+                r->state.store(2);
+                stats_.requests++;
+                stats_.keys += r->d.n;
+            }
+        }
+        done_cv_.notify_all();
+        (void)hipEventDestroy(inflight.front().first);
+        inflight.pop_front();
+    }
+
+    void loop() {
+        std::deque<std::pair<hipEvent_t, std::vector<std::shared_ptr<Request>>>> inflight;
+        int b = 0;
+        std::uint64_t step_no = 0;
+        while (true) {
+            std::vector<std::shared_ptr<Request>> step;
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                if (inflight.empty()) cv_.wait(g, [&] { return stop_ || !pending_.empty(); });
+                if (stop_ && pending_.empty() && inflight.empty()) break;
+                // at most two steps in flight: descriptors are double-buffered
+                if (inflight.size() < 2) pick(step);
+            }
+            if (!step.empty()) {
+                ++step_no;
+                *num_reqs_h_[b] = std::uint32_t(step.size());
+                for (index_t i = 0; i < cfg_.max_requests; ++i)
+                    desc_h_[b][i] = i < step.size() ? step[i]->d : desc_type{};
+                for (auto& r : step) {
+                    r->step = step_no;
+                    r->state.store(1);
+                }
+                launch(b);
+                hipEvent_t ev = nullptr;
+                (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+                (void)hipEventRecord(ev, stream_);
+                inflight.emplace_back(ev, std::move(step));
+                b ^= 1;
+                std::lock_guard<std::mutex> g(mu_);
+                stats_.steps++;
+            }
+            // retire finished steps; with a free descriptor buffer, go back
+            // and build the next step as soon as requests are pending
+            while (!inflight.empty()) {
+                if (hipEventQuery(inflight.front().first) == hipSuccess) {
+                    retire(inflight);
+                    continue;
+                }
+                if (inflight.size() < 2) {
+                    std::unique_lock<std::mutex> g(mu_);
+                    if (!pending_.empty() || stop_) break;
+                    cv_.wait_for(g, std::chrono::microseconds(20), [&] { return !pending_.empty() || stop_; });
+                    if (!pending_.empty() || stop_) break;
+                    continue;
+                }
+                (void)hipEventSynchronize(inflight.front().first);
+                retire(inflight);
+            }
+        }
+    }
+
+    Config cfg_;
+    bool ok_ = true;
+    std::vector<std::unique_ptr<table_type>> tables_;
+    std::vector<index_t> budget_, reserved_;
+    view_type* views_d_ = nullptr;
+    desc_type* desc_d_[2] = {nullptr, nullptr};
+    desc_type* desc_h_[2] = {nullptr, nullptr};
+    std::uint32_t* num_reqs_h_[2] = {nullptr, nullptr};
+    std::uint32_t* num_reqs_d_ = nullptr;
+    std::uint32_t* item_req_ = nullptr;
+    index_t *items_ = nullptr, *first_ = nullptr, *total_ = nullptr, *counter_ = nullptr, *parts_ = nullptr;
+    hipStream_t stream_ = nullptr;
+    hipStream_t side_ = nullptr;
+    Graph graphs_[2];
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::deque<std::shared_ptr<Request>> pending_;
+    std::vector<std::shared_ptr<Request>> all_;
+    index_t next_id_ = 0;
+    bool stop_ = false;
+    Stats stats_;
+    std::thread worker_;
+};
+
+}  // namespace serving
+}  // namespace warpcore

@@ -19,6 +19,7 @@
 #include "defaults.hpp"
 #include "batched_probing.hpp"
 #include "lane_probing.hpp"
+#include "lockstep_probing.hpp"
 #include "engine.hpp"
 #include "probing_schemes.hpp"
 #include "status.hpp"
@@ -372,6 +373,14 @@ public:
                 return;
             }
         }
+        if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+            if (detail::lockstep_inserts()) {
+                detail::launch_insert_lockstep<view_type, StatusHandler,
+                                               detail::lockstep_insert_keys<unsigned(cg_size())>(), true>(
+                    keys_in, values_in, num_in, view(), probing_length, status_out, stream);
+                return;
+            }
+        }
         kernels::insert_batched<view_type, StatusHandler, detail::keys_per_group(), true>
             <<<detail::grid_for_batched(num_in, cg_size()), block_size(), 0, stream>>>(
                 keys_in, values_in, num_in, view(), probing_length, status_out);
@@ -389,6 +398,13 @@ public:
                 kernels::retrieve_lanes<view_type, StatusHandler, detail::keys_per_lane()>
                     <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, num_in, values_out, view(),
                                                                                  probing_length, status_out);
+                return;
+            }
+        }
+        if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+            if (detail::lockstep_lookups()) {
+                detail::launch_retrieve_lockstep<view_type, StatusHandler, detail::lockstep_keys<unsigned(cg_size())>()>(
+                    keys_in, num_in, values_out, view(), probing_length, status_out, stream);
                 return;
             }
         }

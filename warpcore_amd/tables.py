@@ -592,3 +592,162 @@ class BucketListHashTable:
 
     def cg_size(self) -> int:
         return self._fn("cg_size")(self._h)
+
+
+class TableBatch:
+    """Many small SingleValueHashTables in one arena (no reference
+    counterpart; see include/warpcore_c.h). Keys of all tables are passed
+    concatenated with a device array of num_tables + 1 offsets."""
+
+    _NAMES = {(32, 32): "u32u32", (64, 64): "u64u64"}
+
+    def __init__(self, min_capacities, key_bits: int = 32, value_bits: int = 32, seed: int = DEFAULT_SEED,
+                 cg_size: int = 16):
+        import numpy as np
+        self.key_bits, self.value_bits = key_bits, value_bits
+        self._p = f"wc_batch_{self._NAMES[(key_bits, value_bits)]}_"
+        self._lib = _capi.lib()
+        caps = np.ascontiguousarray(min_capacities, dtype=np.uint64)
+        self._h = self._fn("create")(caps.ctypes.data_as(ctypes.c_void_p), len(caps), seed, cg_size)
+        if not self._h:
+            raise ValueError(f"unsupported configuration (cg_size={cg_size})")
+
+    def _fn(self, name):
+        return getattr(self._lib, self._p + name)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._fn("destroy")(h)
+
+    def init(self, stream=None):
+        _check(self._fn("init")(self._h, _stream(stream)), "init")
+
+    def insert(self, keys_in, values_in, offsets, stream=None, probing_length=DEFAULT_PROBING_LENGTH,
+               status_out=None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(values_in, self.value_bits, "values_in")
+        _width(offsets, 64, "offsets")
+        _check(self._fn("insert")(self._h, _ptr(keys_in), _ptr(values_in), _ptr(offsets), keys_in.numel(),
+                                  _stream(stream), probing_length, _ptr(status_out)), "insert")
+
+    def retrieve(self, keys_in, offsets, values_out, stream=None, probing_length=DEFAULT_PROBING_LENGTH,
+                 status_out=None):
+        _width(keys_in, self.key_bits, "keys_in")
+        _width(values_out, self.value_bits, "values_out")
+        _check(self._fn("retrieve")(self._h, _ptr(keys_in), _ptr(offsets), keys_in.numel(), _ptr(values_out),
+                                    _stream(stream), probing_length, _ptr(status_out)), "retrieve")
+
+    def set_launch(self, keys_per_batch: int, item_keys: int, blocks_per_cu: int):
+        _check(self._fn("set_launch")(self._h, keys_per_batch, item_keys, blocks_per_cu), "set_launch")
+
+    def get_launch(self):
+        out = (ctypes.c_uint64 * 3)()
+        _check(self._fn("get_launch")(self._h, out), "get_launch")
+        return tuple(out)
+
+    def tune(self, keys_in, values_in, offsets, queries, query_offsets, values_out, stream=None):
+        out = (ctypes.c_uint64 * 3)()
+        ms = ctypes.c_double(0)
+        _check(self._fn("tune")(self._h, _ptr(keys_in), _ptr(values_in), _ptr(offsets), keys_in.numel(),
+                                _ptr(queries), _ptr(query_offsets), queries.numel(), _ptr(values_out),
+                                _stream(stream), out, ctypes.byref(ms)), "tune")
+        return tuple(out), ms.value
+
+    def status_words(self, stream=None):
+        out = torch.empty(self.num_tables(), dtype=torch.int32, device="cuda")
+        _check(self._fn("copy_status_words")(self._h, _ptr(out), _stream(stream)), "copy_status_words")
+        return out
+
+    def num_tables(self) -> int:
+        return self._fn("num_tables")(self._h)
+
+    def capacity(self, table: int) -> int:
+        return self._fn("capacity")(self._h, table)
+
+    def total_slots(self) -> int:
+        return self._fn("total_slots")(self._h)
+
+    def bytes_total(self) -> int:
+        return self._fn("bytes_total")(self._h)
+
+    def cg_size(self) -> int:
+        return self._fn("cg_size")(self._h)
+
+
+class Server:
+    """Serving loop over device-resident SingleValueHashTables (no
+    reference counterpart; see include/warpcore_c.h)."""
+
+    _NAMES = {(64, 64): "u64u64", (32, 32): "u32u32"}
+    INSERT, RETRIEVE, ERASE = 0, 1, 2
+
+    def __init__(self, table_capacities, key_bits: int = 64, value_bits: int = 64, max_requests: int = 0,
+                 item_keys: int = 0, max_load: float = 0.9, use_graph: bool = True, blocks_per_cu: int = 0):
+        import numpy as np
+        self.key_bits, self.value_bits = key_bits, value_bits
+        self._p = f"wc_server_{self._NAMES[(key_bits, value_bits)]}_"
+        self._lib = _capi.lib()
+        caps = np.ascontiguousarray(table_capacities, dtype=np.uint64)
+        self._h = self._fn("create")(caps.ctypes.data_as(ctypes.c_void_p), len(caps), max_requests, item_keys,
+                                     max_load, int(use_graph), blocks_per_cu)
+        if not self._h:
+            raise ValueError("server could not be created")
+
+    def _fn(self, name):
+        return getattr(self._lib, self._p + name)
+
+    def __del__(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._fn("destroy")(h)
+
+    def submit(self, op: int, table: int, keys_in, values_in=None, values_out=None, status_out=None) -> int:
+        rid = self._fn("submit")(self._h, op, table, _ptr(keys_in), _ptr(values_in), _ptr(values_out),
+                                 _ptr(status_out), keys_in.numel())
+        if rid < 0:
+            raise ValueError("request refused at submission")
+        return rid
+
+    def wait(self, rid: int) -> bool:
+        return self._fn("wait")(self._h, rid) == 1
+
+    def drain(self):
+        _check(self._fn("drain")(self._h), "drain")
+
+    def latency_us(self, rid: int) -> float:
+        return self._fn("latency_us")(self._h, rid)
+
+    def step_of(self, rid: int) -> int:
+        return self._fn("step_of")(self._h, rid)
+
+    def stats(self):
+        out = (ctypes.c_uint64 * 3)()
+        _check(self._fn("stats")(self._h, out), "stats")
+        return {"requests": out[0], "steps": out[1], "keys": out[2]}
+
+    def table_size(self, table: int) -> int:
+        return self._fn("table_size")(self._h, table)
+
+    def table_capacity(self, table: int) -> int:
+        return self._fn("table_capacity")(self._h, table)
+
+    def table_status(self, table: int) -> Status:
+        return Status(self._fn("table_status")(self._h, table))
+
+    def drive(self, mode: int, ops, tables, sizes, arrival_us, keys, values, values_out, status_out=None):
+        """Replay a request list (numpy ops/tables/sizes/arrival_us; device
+        keys/values/values_out/status_out concatenated). mode 0: server,
+        mode 1: one plain launch per request. Returns (latencies_us, makespan_us)."""
+        import numpy as np
+        ops = np.ascontiguousarray(ops, np.uint32)
+        tables = np.ascontiguousarray(tables, np.uint32)
+        sizes = np.ascontiguousarray(sizes, np.uint64)
+        arrival = np.ascontiguousarray(arrival_us, np.float64)
+        lat = np.zeros(len(ops), np.float64)
+        mk = ctypes.c_double(0)
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        _check(self._fn("drive")(self._h, mode, vp(ops), vp(tables), vp(sizes), vp(arrival), len(ops), _ptr(keys),
+                                 _ptr(values), _ptr(values_out), _ptr(status_out), vp(lat), ctypes.byref(mk)),
+               "drive")
+        return lat, mk.value
