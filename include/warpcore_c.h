@@ -574,4 +574,11 @@ WC_API uint64_t wc_get_lane_table_bytes(void);
 WC_API void wc_set_group_kernel(int kernel);
 WC_API int wc_get_group_kernel(void);
 
+/* CountingHashTable bulk inserts of at least 4096 keys (no reference
+ * counterpart): 1 = equal keys of each tile of 1024 occurrences are combined
+ * in LDS and inserted once with their count (default), 0 = every occurrence
+ * inserted on its own. Counts and statuses are the same either way. */
+WC_API void wc_set_counting_combine(int mode);
+WC_API int wc_get_counting_combine(void);
+
 #endif /* WARPCORE_C_H */

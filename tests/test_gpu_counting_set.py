@@ -29,11 +29,20 @@ def _zipf_keys(n, distinct, s, seed, bits=64):
     return from_dev(out, dtype(bits))
 
 
+@pytest.fixture(params=[1, 0], ids=["combine", "each"])
+def combine(request):
+    lib = _lib()
+    saved = lib.wc_get_counting_combine()
+    lib.wc_set_counting_combine(request.param)
+    yield request.param
+    lib.wc_set_counting_combine(saved)
+
+
 @pytest.mark.parametrize("kb,cg,s", [(64, 4, 1.1), (64, 8, 1.0), (32, 4, 1.1), (64, 16, 0.8), (64, 64, 1.1)])
-def test_counting_zipf_exact_counts(cuda, shape, oracle, kb, cg, s):
+def test_counting_zipf_exact_counts(cuda, shape, combine, oracle, kb, cg, s):
     from warpcore_amd.tables import CountingHashTable
     n, distinct = 1 << 20, 1 << 14
-    print(f"[counting] kb={kb} cg={cg} s={s}", flush=True)
+    print(f"[counting] kb={kb} cg={cg} s={s} combine={combine}", flush=True)
     keys = _zipf_keys(n, distinct, s, seed=0xC3 + cg, bits=kb)
     uniq, cnt = np.unique(keys, return_counts=True)
     t = CountingHashTable(-(-distinct * 10 // 9), kb, cg_size=cg)
@@ -62,7 +71,7 @@ def test_counting_zipf_exact_counts(cuda, shape, oracle, kb, cg, s):
     assert np.all(got[len(uniq):] == 0x1234)   # misses leave the output untouched
 
 
-def test_counting_max_count_and_retrieve_all(cuda, shape, oracle):
+def test_counting_max_count_and_retrieve_all(cuda, shape, combine, oracle):
     from warpcore_amd.tables import CountingHashTable
     keys = np.repeat(np.arange(1, 501, dtype=np.uint64), np.arange(1, 501) % 9 + 1)
     np.random.default_rng(3).shuffle(keys)

@@ -85,3 +85,8 @@ extern "C" uint64_t wc_get_lane_table_bytes(void) { return detail::lane_table_by
 // state machine, 1 = lockstep batches.
 extern "C" void wc_set_group_kernel(int kernel) { detail::set_group_kernel(kernel); }
 extern "C" int wc_get_group_kernel(void) { return detail::group_kernel(); }
+
+// CountingHashTable: combine equal keys of a batch before inserting
+// (1, default) or insert every occurrence on its own (0).
+extern "C" void wc_set_counting_combine(int mode) { detail::set_counting_combine(mode); }
+extern "C" int wc_get_counting_combine(void) { return detail::counting_combine(); }

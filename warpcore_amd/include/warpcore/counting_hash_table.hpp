@@ -7,13 +7,16 @@
 // old count was the type's maximum), flagged index_overflow and clamped.
 #pragma once
 
+#include "scan.hpp"
 #include "single_value_hash_table.hpp"
 
 namespace warpcore {
 
 namespace detail {
 
-template<class Base>
+// AddValue: the value handed to finish_insert is the increment (a count of
+// occurrences combined before the insert) instead of 1.
+template<class Base, bool AddValue = false>
 struct CountingView {
     using key_type = typename Base::key_type;
     using value_type = typename Base::value_type;
@@ -53,11 +56,12 @@ struct CountingView {
     }
     // counting_hash_table.cuh:154-170
     template<class Join>
-    WC_D void finish_insert(index_type slot, const value_type&, bool, Status& status, Join& join) const noexcept {
+    WC_D void finish_insert(index_type slot, const value_type& inc, bool, Status& status, Join& join) const noexcept {
+        const value_type add = AddValue ? inc : value_type(1);
         value_type* v = &(base.slots[slot].value);
         if (atomic_load_relaxed(v) < max_count) {
-            const value_type old = atomic_add(v, value_type(1));
-            if (old == std::numeric_limits<value_type>::max()) {
+            const value_type old = atomic_add(v, add);
+            if (old > std::numeric_limits<value_type>::max() - add) {   // the counter wrapped
                 status += Status::index_overflow();
                 join(Status::index_overflow());
                 atomic_exch(v, max_count);
@@ -104,6 +108,150 @@ struct CountingView {
     }
 };
 
+// Combining counts before the insert (see CountingHashTable::insert): a
+// whole-table process setting, 1 = on (default), 0 = every occurrence is
+// inserted on its own.
+inline std::atomic<int>& counting_combine_setting() noexcept {
+    static std::atomic<int> m{1};
+    return m;
+}
+WC_H int counting_combine() noexcept { return counting_combine_setting().load(std::memory_order_relaxed); }
+WC_H void set_counting_combine(int m) noexcept { counting_combine_setting().store(m, std::memory_order_relaxed); }
+
+constexpr unsigned kCombineTile = 1024;             // occurrences per block and tile
+constexpr unsigned kCombineSlots = 2 * kCombineTile;  // LDS hash slots (load <= 1/2)
+
+}  // namespace detail
+
+namespace kernels {
+
+// One tile of 1024 occurrences per block: identical keys are combined in an
+// LDS hash map (CAS on the key, atomicAdd on the count, atomicMin on the
+// first occurrence), then the distinct keys of the tile are appended to
+// keys_out / counts_out with rep_out = their first occurrence, and every
+// occurrence gets pos[i] = its entry. Invalid keys (EMPTY, TOMBSTONE) get an
+// entry each, so the insert reports them.
+template<class K, class V, K EmptyKey, K TombstoneKey>
+__global__ __launch_bounds__(256) void count_combine(const K* __restrict__ keys_in, index_t num_in,
+                                                     K* __restrict__ keys_out, V* __restrict__ counts_out,
+                                                     std::uint32_t* __restrict__ rep_out,
+                                                     std::uint32_t* __restrict__ pos, index_t* __restrict__ num_out) {
+    constexpr unsigned T = detail::kCombineTile, S = detail::kCombineSlots, P = T / 256, Q = S / 256;
+    __shared__ K lkey[S];
+    __shared__ std::uint32_t lcnt[S], lrep[S], lent[S];
+    __shared__ index_t base_sh;
+    for (index_t tile = index_t(blockIdx.x) * T; tile < num_in; tile += index_t(gridDim.x) * T) {
+        for (unsigned q = 0; q < Q; ++q) {
+            const unsigned s = q * 256 + threadIdx.x;
+            lkey[s] = EmptyKey;
+            lcnt[s] = 0;
+            lrep[s] = ~0u;
+        }
+        __syncthreads();
+        K key[P];
+        unsigned slot[P];   // LDS slot of the key, ~0u = invalid key, ~1u = past the end
+        unsigned invalid = 0;
+#pragma unroll
+        for (unsigned j = 0; j < P; ++j) {
+            const index_t i = tile + j * 256 + threadIdx.x;
+            slot[j] = ~1u;
+            if (i >= num_in) continue;
+            key[j] = keys_in[i];
+            if (key[j] == EmptyKey || key[j] == TombstoneKey) {
+                slot[j] = ~0u;
+                ++invalid;
+                continue;
+            }
+            unsigned h = unsigned(hashers::MurmurHash<K>::hash(key[j])) & (S - 1);
+            while (true) {
+                const K prev = detail::atomic_cas(lkey + h, EmptyKey, key[j]);
+                if (prev == EmptyKey || prev == key[j]) break;
+                h = (h + 1) & (S - 1);
+            }
+            atomicAdd(lcnt + h, 1u);
+            atomicMin(lrep + h, unsigned(j * 256 + threadIdx.x));
+            slot[j] = h;
+        }
+        __syncthreads();
+        // entries of this thread: its Q LDS slots that hold a key, then its invalid keys
+        unsigned mine = invalid;
+        for (unsigned q = 0; q < Q; ++q) mine += lkey[threadIdx.x * Q + q] != EmptyKey;
+        index_t total;
+        const index_t incl = detail::block_inclusive_scan(index_t(mine), total);
+        if (threadIdx.x == 0) base_sh = detail::atomic_add(num_out, total);
+        __syncthreads();
+        index_t e = base_sh + incl - mine;
+        for (unsigned q = 0; q < Q; ++q) {
+            const unsigned s = threadIdx.x * Q + q;
+            if (lkey[s] != EmptyKey) {
+                keys_out[e] = lkey[s];
+                counts_out[e] = V(lcnt[s]);
+                rep_out[e] = std::uint32_t(tile + lrep[s]);
+                lent[s] = std::uint32_t(e);
+                ++e;
+            }
+        }
+#pragma unroll
+        for (unsigned j = 0; j < P; ++j) {
+            if (slot[j] != ~0u) continue;
+            const index_t i = tile + j * 256 + threadIdx.x;
+            keys_out[e] = key[j];
+            counts_out[e] = V(1);
+            rep_out[e] = std::uint32_t(i);
+            pos[i] = std::uint32_t(e);
+            ++e;
+        }
+        __syncthreads();
+#pragma unroll
+        for (unsigned j = 0; j < P; ++j)
+            if (slot[j] < S) pos[tile + j * 256 + threadIdx.x] = lent[slot[j]];
+        __syncthreads();
+    }
+}
+
+// Join duplicate_key for keys that occurred more than once in the batch
+// and were stored (the reference joins it for every duplicate occurrence).
+template<class V>
+__global__ __launch_bounds__(256) void count_join_duplicates(const V* __restrict__ counts,
+                                                             const Status* __restrict__ entry_status,
+                                                             const index_t* __restrict__ num_entries,
+                                                             Status* __restrict__ word) {
+    detail::DeferredJoin join;
+    const index_t n = *num_entries;
+    const index_t stride = index_t(gridDim.x) * blockDim.x;
+    for (index_t e = index_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += stride) {
+        const Status s = entry_status[e];
+        if (counts[e] > 1 && !s.has_any(~(Status::duplicate_key() + Status::index_overflow())))
+            join(Status::duplicate_key());
+    }
+    join.flush(word);
+}
+
+// Per occurrence: its entry's status for the first occurrence of a key;
+// for the others duplicate_key where the entry was stored (none or
+// duplicate), else the entry's status (the table was full, ...).
+template<class StatusHandler>
+__global__ __launch_bounds__(256) void count_expand(const std::uint32_t* __restrict__ pos,
+                                                    const std::uint32_t* __restrict__ rep,
+                                                    const Status* __restrict__ entry_status, index_t first,
+                                                    index_t num_in,
+                                                    typename StatusHandler::base_type* __restrict__ status_out) {
+    const index_t stride = index_t(gridDim.x) * blockDim.x;
+    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < num_in; i += stride) {
+        const std::uint32_t e = pos[i];
+        Status s = entry_status[e];
+        if (rep[e] != std::uint32_t(i)) {
+            const Status stored = s - Status::index_overflow();
+            if (!stored.has_any(~Status::duplicate_key())) s = Status::duplicate_key() + (s - stored);
+        }
+        StatusHandler::handle(s, status_out, first + i);
+    }
+}
+
+}  // namespace kernels
+
+namespace detail {
+
 }  // namespace detail
 
 template<class Key, class Value = index_t, Key EmptyKey = defaults::empty_key<Key>(),
@@ -122,6 +270,7 @@ public:
     using index_type = index_t;
     using status_type = Status;
     using view_type = detail::CountingView<typename base_type::view_type>;
+    using add_view_type = detail::CountingView<typename base_type::view_type, true>;
 
     WC_HD static constexpr key_type empty_key() noexcept { return EmptyKey; }
     WC_HD static constexpr key_type tombstone_key() noexcept { return TombstoneKey; }
@@ -135,10 +284,16 @@ public:
         if (!no_init) init(seed);
     }
     WC_HD CountingHashTable(const CountingHashTable& o) noexcept
-        : base_table_(o.base_table_), max_count_(o.max_count_), is_copy_(true) {}
+        : base_table_(o.base_table_), max_count_(o.max_count_), is_copy_(true), scratch_(o.scratch_) {}
     WC_H CountingHashTable(CountingHashTable&& o) noexcept
-        : base_table_(std::move(o.base_table_)), max_count_(o.max_count_), is_copy_(o.is_copy_) {
+        : base_table_(std::move(o.base_table_)), max_count_(o.max_count_), is_copy_(o.is_copy_),
+          scratch_(o.scratch_) {
         o.is_copy_ = true;
+    }
+    WC_HD ~CountingHashTable() noexcept {
+#if !defined(__HIP_DEVICE_COMPILE__)
+        if (!is_copy_) scratch_.release();
+#endif
     }
 
     // counting_hash_table.cuh:113-124; the EMPTY fill also zeroes the counters
@@ -161,32 +316,83 @@ public:
         return view().erase(key_in, group, probing_length, join);
     }
 
-    // counting_hash_table.cuh:182-194
+    // counting_hash_table.cuh:182-194. Batches of at least 4096 occurrences
+    // are first combined (count_combine: one insert of n per distinct key and
+    // tile of 1024 occurrences), so a hot key costs one atomicAdd per tile
+    // instead of one per occurrence; counts, statuses (one `none` per new
+    // key, duplicate_key for the other occurrences) and the table status are
+    // those of per-occurrence inserts.
     template<class StatusHandler = defaults::status_handler_t>
     WC_H void insert(const key_type* keys_in, index_type num_in, hipStream_t stream = 0,
                      index_type probing_length = defaults::probing_length(),
                      typename StatusHandler::base_type* status_out = nullptr) noexcept {
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!base_table_.is_initialized() || num_in == 0) return;
+        if (detail::counting_combine() == 1 && num_in >= 4096 && num_in < (index_type(1) << 32)) {
+            insert_combined<StatusHandler>(keys_in, num_in, stream, probing_length, status_out);
+            return;
+        }
+        insert_each<view_type, StatusHandler, false>(keys_in, nullptr, num_in, view(), stream, probing_length,
+                                                     status_out);
+    }
+
+    // every occurrence inserted on its own (the reference's kernel shape)
+    template<class View, class StatusHandler, bool HasValues>
+    WC_H void insert_each(const key_type* keys_in, const value_type* values_in, index_type num_in, const View& v,
+                          hipStream_t stream, index_type probing_length,
+                          typename StatusHandler::base_type* status_out) noexcept {
         if constexpr (detail::LaneWindow<typename view_type::slot_type, cg_size()>::usable) {
             if (capacity() * sizeof(typename view_type::slot_type) <= detail::lane_table_bytes()) {
-                kernels::insert_lanes<view_type, StatusHandler, detail::keys_per_lane(), false>
-                    <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, nullptr, num_in, view(),
+                kernels::insert_lanes<View, StatusHandler, detail::keys_per_lane(), HasValues>
+                    <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, v,
                                                                                  probing_length, status_out);
                 return;
             }
         }
-        if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+        if constexpr (detail::fixed_step_scheme<typename View::probing_scheme_type>::value) {
             if (detail::lockstep_inserts()) {
-                detail::launch_insert_lockstep<view_type, StatusHandler,
-                                               detail::lockstep_insert_keys<unsigned(cg_size())>(), false>(
-                    keys_in, nullptr, num_in, view(), probing_length, status_out, stream);
+                detail::launch_insert_lockstep<View, StatusHandler,
+                                               detail::lockstep_insert_keys<unsigned(cg_size())>(), HasValues>(
+                    keys_in, values_in, num_in, v, probing_length, status_out, stream);
                 return;
             }
         }
-        kernels::insert_batched<view_type, StatusHandler, detail::keys_per_group(), false>
+        kernels::insert_batched<View, StatusHandler, detail::keys_per_group(), HasValues>
             <<<detail::grid_for_batched(num_in, cg_size()), block_size(), 0, stream>>>(
-                keys_in, nullptr, num_in, view(), probing_length, status_out);
+                keys_in, values_in, num_in, v, probing_length, status_out);
+    }
+
+    template<class StatusHandler>
+    WC_H void insert_combined(const key_type* keys_in, index_type num_in, hipStream_t stream,
+                              index_type probing_length, typename StatusHandler::base_type* status_out) noexcept {
+        constexpr index_type chunk = index_type(1) << 24;
+        const index_type cap = num_in < chunk ? num_in : chunk;
+        if (!scratch_.reserve(cap)) {   // no memory for the combining buffers: insert each occurrence
+            insert_each<view_type, StatusHandler, false>(keys_in, nullptr, num_in, view(), stream, probing_length,
+                                                         status_out);
+            return;
+        }
+        for (index_type first = 0; first < num_in; first += cap) {
+            const index_type n = num_in - first < cap ? num_in - first : cap;
+            storage::kernels::fill<index_type><<<1, 64, 0, stream>>>(scratch_.num, 1, index_type(0));
+            const index_type tiles = SDIV(n, index_type(detail::kCombineTile));
+            const index_type limit = index_type(detail::num_compute_units()) * 4;
+            kernels::count_combine<key_type, value_type, EmptyKey, TombstoneKey>
+                <<<unsigned(tiles < limit ? tiles : limit), 256, 0, stream>>>(
+                    keys_in + first, n, scratch_.keys, scratch_.counts, scratch_.rep, scratch_.pos, scratch_.num);
+            index_type entries = 0;
+            (void)hipMemcpyAsync(&entries, scratch_.num, sizeof(index_type), hipMemcpyDeviceToHost, stream);
+            (void)hipStreamSynchronize(stream);
+            insert_each<add_view_type, status_handlers::ReturnStatus, true>(
+                scratch_.keys, scratch_.counts, entries, add_view(), stream, probing_length, scratch_.status);
+            kernels::count_join_duplicates<value_type>
+                <<<detail::grid_for(entries, 1, 4), block_size(), 0, stream>>>(scratch_.counts, scratch_.status,
+                                                                            scratch_.num, base_table_.status_);
+            if constexpr (!std::is_same<StatusHandler, status_handlers::ReturnNothing>::value)
+                if (status_out != nullptr)
+                    kernels::count_expand<StatusHandler><<<detail::grid_for(n, 1, 8), block_size(), 0, stream>>>(
+                        scratch_.pos, scratch_.rep, scratch_.status, first, n, status_out);
+        }
     }
 
     // counting_hash_table.cuh:236-249
@@ -250,11 +456,50 @@ public:
     WC_HD bool is_copy() const noexcept { return is_copy_; }
 
     WC_HD view_type view() const noexcept { return view_type{base_table_.view(), max_count_}; }
+    WC_HD add_view_type add_view() const noexcept { return add_view_type{base_table_.view(), max_count_}; }
 
 private:
+    // device buffers of insert_combined, grown on demand
+    struct Scratch {
+        key_type* keys = nullptr;
+        value_type* counts = nullptr;
+        std::uint32_t* rep = nullptr;
+        std::uint32_t* pos = nullptr;
+        Status* status = nullptr;
+        index_type* num = nullptr;
+        index_type capacity = 0;
+        WC_H bool reserve(index_type n) noexcept {
+            if (n <= capacity) return true;
+            release();
+            bool ok = hipMalloc(reinterpret_cast<void**>(&keys), sizeof(key_type) * n) == hipSuccess &&
+                      hipMalloc(reinterpret_cast<void**>(&counts), sizeof(value_type) * n) == hipSuccess &&
+                      hipMalloc(reinterpret_cast<void**>(&rep), sizeof(std::uint32_t) * n) == hipSuccess &&
+                      hipMalloc(reinterpret_cast<void**>(&pos), sizeof(std::uint32_t) * n) == hipSuccess &&
+                      hipMalloc(reinterpret_cast<void**>(&status), sizeof(Status) * n) == hipSuccess &&
+                      hipMalloc(reinterpret_cast<void**>(&num), sizeof(index_type)) == hipSuccess;
+            if (!ok) {
+                release();
+                return false;
+            }
+            capacity = n;
+            return true;
+        }
+        WC_H void release() noexcept {
+            for (void* p : {(void*)keys, (void*)counts, (void*)rep, (void*)pos, (void*)status, (void*)num})
+                if (p) (void)hipFree(p);
+            keys = nullptr;
+            counts = nullptr;
+            rep = pos = nullptr;
+            status = nullptr;
+            num = nullptr;
+            capacity = 0;
+        }
+    };
+
     base_type base_table_;
     value_type max_count_;
     bool is_copy_;
+    Scratch scratch_;
 };
 
 }  // namespace warpcore
