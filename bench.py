@@ -249,7 +249,7 @@ def run_batched(g: Gpu, num_tables: int = 65536, steps: int = 3, warmup: int = 1
 
 
 def run_serving(g: Gpu, num_tables: int = 64, total_bytes: float = 100e9, num_requests: int = 2000,
-                load: float = 0.5, seed: int = 0xC8):
+                load: float = 0.5, seed: int = 0xC8, server_kw=None):
     """Open-loop request stream against 64 resident SingleValueHashTable
     <u64,u64> (about 100 GB of slots): sizes log-uniform 2^8..2^22 keys,
     30 % insert / 60 % retrieve / 10 % erase, uniform u64 keys with 20 %
@@ -280,7 +280,7 @@ def run_serving(g: Gpu, num_tables: int = 64, total_bytes: float = 100e9, num_re
     res = {"tables": num_tables, "table_slots": int(caps[0]), "requests": num_requests, "keys": n}
 
     def one(mode, arrival):
-        srv = Server(caps, 64, 64)
+        srv = Server(caps, 64, 64, **(server_kw or {}))
         lat, mk = srv.drive(mode, ops, tabs, sizes, arrival, keys, values, out, st)
         steps = srv.stats()["steps"] if mode == 0 else num_requests
         del srv
@@ -387,11 +387,12 @@ def main():
             log(f"[bench] serving: {extras['serving_64x1.56GB_u64u64']}")
     if args.calibrate_copy_mib:
         torch = g.torch
+        # wc_util_copy: 16 bytes per lane and access, each byte read and written once
         a = torch.empty(args.calibrate_copy_mib << 20, dtype=torch.uint8, device="cuda")
         b = torch.empty_like(a)
         a.fill_(1)
         torch.cuda.synchronize()
-        b.copy_(a)
+        g.check(g.lib.wc_util_copy(g.p(b), g.p(a), a.numel(), None), "calibration copy")
         torch.cuda.synchronize()
         log(f"[bench] calibration copy of {args.calibrate_copy_mib} MiB done")
 

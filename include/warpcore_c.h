@@ -487,13 +487,16 @@ WC_API uint32_t wc_batch_u64u64_cg_size(void* batch);
  * retrieve (op 1) and erase (op 2) requests from many host threads. A
  * scheduler thread coalesces admitted requests into steps: one persistent
  * launch over all requests of a step, captured as a HIP graph when
- * use_graph. Per table, requests run in arrival order (a step holds a run
- * of lookups or one insert/erase per table). An insert that does not fit
- * its table's slot budget (max_load * capacity, never given back) is
- * refused: every key reports out_of_memory and wait returns 0.
+ * use_graph. A step holds at most max_step_keys keys; a larger request
+ * runs in chunks over consecutive steps. Per table, requests run in
+ * arrival order (a step holds a run of lookups or one insert/erase, or one
+ * chunk of it, per table). An insert that does not fit its table's slot
+ * budget (max_load * capacity, never given back) is refused: every key
+ * reports out_of_memory and wait returns 0.
  *   create       table_capacities: HOST array (minimum capacities);
  *                max_requests per step (0 = 1024), item_keys per work item
- *                (0 = 1024), max_load (<= 0: 0.9), blocks_per_cu (0 = 8)
+ *                (0 = 64), max_step_keys (0 = 2^24), max_load (<= 0: 0.9),
+ *                blocks_per_cu (0 = 8)
  *   submit       device buffers, kept valid until the request is done;
  *                returns the request id (-1 on a bad argument)
  *   wait / drain block until one / every request is done
@@ -508,7 +511,7 @@ WC_API uint32_t wc_batch_u64u64_cg_size(void* batch);
  *                *makespan_us
  * ---------------------------------------------------------------------- */
 /* Server over SingleValueHashTable<uint64_t, uint64_t> */
-WC_API void* wc_server_u64u64_create(const uint64_t* table_capacities, uint64_t num_tables, uint64_t max_requests, uint64_t item_keys, float max_load, int use_graph, int blocks_per_cu);
+WC_API void* wc_server_u64u64_create(const uint64_t* table_capacities, uint64_t num_tables, uint64_t max_requests, uint64_t item_keys, uint64_t max_step_keys, float max_load, int use_graph, int blocks_per_cu);
 WC_API void wc_server_u64u64_destroy(void* server);
 WC_API int64_t wc_server_u64u64_submit(void* server, uint32_t op, uint32_t table, const uint64_t* keys_in, const uint64_t* values_in, uint64_t* values_out, uint32_t* status_out, uint64_t num_in);
 WC_API int wc_server_u64u64_wait(void* server, int64_t id);
@@ -522,7 +525,7 @@ WC_API uint32_t wc_server_u64u64_table_status(void* server, uint32_t table);
 WC_API int wc_server_u64u64_drive(void* server, int mode, const uint32_t* ops, const uint32_t* tables, const uint64_t* sizes, const double* arrival_us, uint64_t num_requests, const uint64_t* keys_in, const uint64_t* values_in, uint64_t* values_out, uint32_t* status_out, double* latency_us_out, double* makespan_us);
 
 /* Server over SingleValueHashTable<uint32_t, uint32_t> */
-WC_API void* wc_server_u32u32_create(const uint64_t* table_capacities, uint64_t num_tables, uint64_t max_requests, uint64_t item_keys, float max_load, int use_graph, int blocks_per_cu);
+WC_API void* wc_server_u32u32_create(const uint64_t* table_capacities, uint64_t num_tables, uint64_t max_requests, uint64_t item_keys, uint64_t max_step_keys, float max_load, int use_graph, int blocks_per_cu);
 WC_API void wc_server_u32u32_destroy(void* server);
 WC_API int64_t wc_server_u32u32_submit(void* server, uint32_t op, uint32_t table, const uint32_t* keys_in, const uint32_t* values_in, uint32_t* values_out, uint32_t* status_out, uint64_t num_in);
 WC_API int wc_server_u32u32_wait(void* server, int64_t id);
@@ -553,6 +556,11 @@ WC_API int wc_gen_zipf_u32(uint32_t* out, uint64_t n, uint64_t num_distinct, dou
 WC_API int wc_gen_zipf_u64(uint64_t* out, uint64_t n, uint64_t num_distinct, double s, uint64_t seed, uint64_t key_offset, void* stream);
 WC_API int wc_gen_interleave_u32(uint32_t* out, const uint32_t* a, const uint32_t* b, uint64_t half, uint64_t seed, void* stream);
 WC_API int wc_gen_interleave_u64(uint64_t* out, const uint64_t* a, const uint64_t* b, uint64_t half, uint64_t seed, void* stream);
+
+/* Device copy of `bytes` (a multiple of 16, 16-byte aligned pointers), 16
+ * bytes per lane: a streaming kernel of known traffic, used to calibrate
+ * memory-traffic counters. No reference counterpart. */
+WC_API int wc_util_copy(void* dst, const void* src, uint64_t bytes, void* stream);
 
 /* Capacity rule detail::get_valid_capacity (include/warpcore/base.cuh:27-34)
  * and the unique-key map, evaluated on the host. */

@@ -79,13 +79,14 @@ def dev_empty(n: int, bits: int, fill: int = 0):
 
 class kernel_shape:
     """Context manager selecting the bulk-kernel shape: 'lanes' (one lane per
-    key), 'groups' (cooperative groups, streaming lookups) or 'lockstep'
-    (cooperative groups, lockstep lookups), by moving the table-size limit
-    of the lane-per-key path and choosing the group lookup kernel."""
+    key), 'groups' (cooperative groups, streaming kernels), 'lockstep'
+    (cooperative groups, lockstep kernels) or 'refill' (cooperative groups,
+    refill inserts and lookups), by moving the table-size limit of
+    the lane-per-key path and choosing the group kernels."""
 
     def __init__(self, shape: str):
         self.limit = (1 << 62) if shape == "lanes" else 0
-        self.group_kernel = 0 if shape == "groups" else 3   # all streaming / all lockstep
+        self.group_kernel = {"groups": 0, "refill": 12}.get(shape, 3)
 
     def __enter__(self):
         from warpcore_amd import _capi

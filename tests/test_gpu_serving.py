@@ -64,15 +64,20 @@ def _run(mode, caps, ops, tables, sizes, arrival, keys, vals, **kw):
     return res
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_server_matches_one_launch_per_request(cuda, use_graph):
+@pytest.mark.parametrize("use_graph,max_step_keys", [(True, 0), (False, 0), (True, 5000)])
+def test_server_matches_one_launch_per_request(cuda, use_graph, max_step_keys):
+    """max_step_keys=5000: requests of up to 2^16 keys run in chunks over
+    several steps."""
     caps = np.full(6, 1 << 20, np.uint64)
     ops, tables, sizes, arrival, keys, vals = _stream(240, len(caps), seed=5)
     out_s, st_s, lat_s, mk_s, size_s, tst_s, stats = _run(0, caps, ops, tables, sizes, arrival, keys, vals,
-                                                          use_graph=use_graph)
+                                                          use_graph=use_graph, max_step_keys=max_step_keys)
     out_p, st_p, lat_p, mk_p, size_p, tst_p, _ = _run(1, caps, ops, tables, sizes, arrival, keys, vals)
     assert stats["requests"] == len(ops) and stats["keys"] == int(sizes.sum())
-    assert 1 <= stats["steps"] <= len(ops)
+    if max_step_keys:
+        assert stats["steps"] >= int(sizes.sum()) // max_step_keys
+    else:
+        assert 1 <= stats["steps"] <= len(ops)
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     for r in range(len(ops)):
         s = slice(off[r], off[r + 1])

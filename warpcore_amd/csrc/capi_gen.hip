@@ -9,7 +9,24 @@ using wc_capi::last_error;
 
 namespace {
 unsigned grid1(index_t n) { return detail::grid_for(n, 1, 8); }
+
+// Contiguous copy, 16 bytes per lane and access: a plain streaming kernel
+// of known traffic (reads and writes `n16` * 16 bytes once).
+__global__ __launch_bounds__(256) void copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, index_t n16) {
+    const index_t stride = index_t(gridDim.x) * blockDim.x;
+    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
 }  // namespace
+
+extern "C" int wc_util_copy(void* dst, const void* src, uint64_t bytes, void* stream) {
+    if (bytes == 0) return 0;
+    if (!dst || !src || bytes % 16 != 0 || reinterpret_cast<std::uintptr_t>(dst) % 16 != 0 ||
+        reinterpret_cast<std::uintptr_t>(src) % 16 != 0)
+        return wc_capi::kBadArgument;
+    copy16<<<grid1(bytes / 16), block_size(), 0, as_stream(stream)>>>(static_cast<uint4*>(dst),
+                                                                      static_cast<const uint4*>(src), bytes / 16);
+    return last_error();
+}
 
 extern "C" int wc_gen_unique_u32(uint32_t* out, uint64_t n, uint64_t offset, void* stream) {
     if (n == 0) return 0;

@@ -350,6 +350,11 @@ public:
             }
         }
         if constexpr (detail::fixed_step_scheme<typename View::probing_scheme_type>::value) {
+            if (detail::refill_inserts()) {
+                detail::launch_insert_refill<View, StatusHandler, detail::refill_insert_keys<unsigned(cg_size())>(), HasValues>(
+                    keys_in, values_in, num_in, v, probing_length, status_out, stream);
+                return;
+            }
             if (detail::lockstep_inserts()) {
                 detail::launch_insert_lockstep<View, StatusHandler,
                                                detail::lockstep_insert_keys<unsigned(cg_size())>(), HasValues>(
@@ -411,6 +416,11 @@ public:
             }
         }
         if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+            if (detail::refill_lookups()) {
+                detail::launch_retrieve_refill<view_type, StatusHandler, detail::refill_keys<unsigned(cg_size())>()>(
+                    keys_in, num_in, values_out, view(), probing_length, status_out, stream);
+                return;
+            }
             if (detail::lockstep_lookups()) {
                 detail::launch_retrieve_lockstep<view_type, StatusHandler, detail::lockstep_keys<unsigned(cg_size())>()>(
                     keys_in, num_in, values_out, view(), probing_length, status_out, stream);

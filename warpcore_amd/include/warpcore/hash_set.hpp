@@ -236,6 +236,11 @@ public:
             }
         }
         if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+            if (detail::refill_inserts()) {
+                detail::launch_insert_refill<view_type, StatusHandler, detail::refill_insert_keys<unsigned(cg_size())>(), false>(
+                    keys_in, nullptr, num_in, view(), probing_length, status_out, stream);
+                return;
+            }
             if (detail::lockstep_inserts()) {
                 detail::launch_insert_lockstep<view_type, StatusHandler,
                                                detail::lockstep_insert_keys<unsigned(cg_size())>(), false>(
@@ -264,6 +269,11 @@ public:
             }
         }
         if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+            if (detail::refill_lookups()) {
+                detail::launch_retrieve_refill<view_type, StatusHandler, detail::refill_keys<unsigned(cg_size())>()>(
+                    keys_in, num_in, flags_out, view(), probing_length, status_out, stream);
+                return;
+            }
             if (detail::lockstep_lookups()) {
                 detail::launch_retrieve_lockstep<view_type, StatusHandler, detail::lockstep_keys<unsigned(cg_size())>()>(
                     keys_in, num_in, flags_out, view(), probing_length, status_out, stream);

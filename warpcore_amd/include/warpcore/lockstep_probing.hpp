@@ -182,6 +182,112 @@ WC_D void retrieve_lockstep_range(const typename Core::key_type* __restrict__ ke
     }
 }
 
+// Erase of keys_in[first], keys_in[first + stride], ... below `end`, U at a
+// time per group (single_value_hash_table.cuh:425-468 per key): the lane
+// holding a hit writes the tombstone over the key; a window with an EMPTY
+// lane and no hit ends in key_not_found; misses and exhausted budgets are
+// returned, only invalid keys are joined. `Core` provides erase_slot().
+template<class Core, class StatusHandler, int U, class W>
+WC_D void erase_lockstep_range(const typename Core::key_type* __restrict__ keys_in, index_t first, index_t end,
+                               index_t stride, const Core& core, index_t budget,
+                               typename StatusHandler::base_type* __restrict__ status_out,
+                               StagedKey<typename Core::key_type, W>* my, DeferredJoin& join) {
+    using K = typename Core::key_type;
+    using Slot = typename Core::probe_slot_type;
+    using Cursor = WindowCursor<typename Core::probing_scheme_type>;
+    using Entry = StagedKey<K, W>;
+    constexpr unsigned CG = unsigned(Core::cg_size());
+    static_assert(U >= 1 && unsigned(U) <= CG, "a batch holds at most one key per lane");
+    const unsigned lane = lane_id();
+    const unsigned rank = lane & (CG - 1);
+    const unsigned shift = lane & ~(CG - 1);
+    constexpr std::uint64_t full = tile<CG>::full_mask();
+    auto gballot = [&](bool p) -> std::uint64_t {
+        const std::uint64_t m = __builtin_amdgcn_ballot_w64(p);
+        if constexpr (CG == 64) return m;
+        else return (m >> shift) & full;
+    };
+    const probing_schemes::Geometry& geo = core.geometry();
+    const W p = W(geo.windows);
+    const Slot* __restrict__ slots = core.slot_ptr();
+
+    for (index_t base = first; base < end; base += stride) {
+        if (rank < unsigned(U)) {
+            const index_t i = base + rank;
+            const K key = keys_in[i < end ? i : end - 1];
+            Entry e;
+            e.key = key;
+            e.w = 0;
+            e.step = 1;
+            if (i >= end) e.state = kPastEnd;
+            else if (!core.ready()) e.state = kNotReady;
+            else if (!core.is_valid_key(key)) e.state = kKeyInvalid;
+            else {
+                Cursor c;
+                c.begin(key, core.hash_seed(), geo);
+                e.w = W(c.w);
+                e.step = W(c.step);
+                e.state = kKeyOk;
+            }
+            my[rank] = e;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+        K key[U];
+        W w[U], step[U];
+        Status st[U];
+        std::uint32_t live = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const Entry e = my[u];
+            key[u] = e.key;
+            w[u] = e.w;
+            step[u] = e.step;
+            st[u] = Status::none();
+            if (e.state == kKeyOk) live |= 1u << u;
+            else if (e.state == kNotReady) st[u] = Status::not_initialized();
+            else if (e.state == kKeyInvalid) {
+                st[u] = Status::invalid_key();
+                join(st[u]);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+
+        for (index_t round = 0; live != 0; ++round) {
+            Slot s[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const W at = (live >> u) & 1u ? w[u] : W(0);
+                s[u] = load_slot(slots + index_t(at) * CG + rank);
+            }
+            const bool last = round + 1 >= budget;
+            std::uint32_t next = 0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool on = (live >> u) & 1u;
+                const K tk = Core::key_of(s[u]);
+                const std::uint64_t hit = gballot(tk == key[u]);
+                const std::uint64_t empty = gballot(Core::is_empty_key(tk));
+                if (on && hit != 0 && rank == unsigned(ffs64(hit) - 1)) core.erase_slot(index_t(w[u]) * CG + rank);
+                if (on && hit == 0 && empty != 0) st[u] = Status::key_not_found();
+                if (on && hit == 0 && empty == 0 && last) st[u] = Status::probing_length_exceeded();
+                if (on && hit == 0 && empty == 0 && !last) {
+                    W nw = w[u] + step[u];
+                    w[u] = nw >= p ? nw - p : nw;
+                    next |= 1u << u;
+                }
+            }
+            live = next;
+        }
+
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (rank == unsigned(u) && base + u < end) StatusHandler::handle(st[u], status_out, base + u);
+    }
+}
+
 // Insert with U keys per group in lockstep (single_value_hash_table.cuh:742-823
 // per key). A key is in one of two phases: `load` (read its current
 // window) or `cas` (its window had EMPTY lanes; the lowest untried one
@@ -377,6 +483,21 @@ __global__ __launch_bounds__(256) void retrieve_lockstep(const typename Core::ke
     join.flush(core.status_word());
 }
 
+template<class Core, class StatusHandler, int U, class W>
+__global__ __launch_bounds__(256) void erase_lockstep(const typename Core::key_type* __restrict__ keys_in,
+                                                      index_t num_in, Core core, index_t probing_length,
+                                                      typename StatusHandler::base_type* __restrict__ status_out) {
+    constexpr unsigned CG = unsigned(Core::cg_size());
+    __shared__ detail::StagedKey<typename Core::key_type, W> stage[256 / CG * U];
+    detail::DeferredJoin join;
+    const index_t gid = (index_t(blockIdx.x) * blockDim.x + threadIdx.x) / CG;
+    const index_t groups = (index_t(gridDim.x) * blockDim.x) / CG;
+    const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, core.geometry().windows);
+    detail::erase_lockstep_range<Core, StatusHandler, U, W>(keys_in, gid * U, num_in, groups * U, core, budget,
+                                                            status_out, stage + (threadIdx.x / CG) * U, join);
+    join.flush(core.status_word());
+}
+
 template<class Core, class StatusHandler, int U, class W, bool HasValues>
 __global__ __launch_bounds__(256) void insert_lockstep(const typename Core::key_type* __restrict__ keys_in,
                                                        const typename Core::value_type* __restrict__ values_in,
@@ -398,31 +519,53 @@ __global__ __launch_bounds__(256) void insert_lockstep(const typename Core::key_
 
 namespace detail {
 
-// Grid of a lockstep launch: one group per U keys, at most 8 blocks of 256
-// per CU (the loop is grid-stride).
+// Grid of a lockstep launch: one group per U keys, at most
+// WARPCORE_LOCKSTEP_BLOCKS_PER_CU blocks of 256 per CU (the loop is
+// grid-stride).
+#ifndef WARPCORE_LOCKSTEP_BLOCKS_PER_CU
+#define WARPCORE_LOCKSTEP_BLOCKS_PER_CU 8
+#endif
 template<unsigned CG, int U>
 WC_H unsigned grid_for_lockstep(index_t items) noexcept {
     const index_t groups = SDIV(items, index_t(U));
     const index_t blocks = SDIV(groups * CG, block_size());
-    const index_t cap = index_t(num_compute_units()) * 8;
+    const index_t cap = index_t(num_compute_units()) * WARPCORE_LOCKSTEP_BLOCKS_PER_CU;
     const index_t g = std::min(blocks, cap);
     return unsigned(g == 0 ? 1 : g);
 }
 
 // Kernels of bulk calls on tables probed by cooperative groups
 // (process-wide setting): bit 0 set = lockstep inserts, bit 1 set =
-// lockstep lookups; a clear bit selects the streaming state machine. The
-// default, lockstep inserts and streaming lookups, is the faster pair on
-// MI355X (lockstep lookups lose to the streaming kernel when half the keys
-// miss: a batch waits for its longest probe sequence).
+// lockstep lookups and erases, bit 2 set = refill lookups (refill_probing.hpp,
+// before bit 1), bit 3 set = refill inserts (before bit 0); a clear bit
+// selects the streaming state machine. The
+// default, refill inserts and lookups, is the fastest measured on MI355X
+// (lockstep batches wait for their longest probe sequence, which at load
+// factor 0.9 is many windows; refill slots do not wait).
 inline std::atomic<int>& group_kernel_setting() noexcept {
-    static std::atomic<int> k{1};
+    static std::atomic<int> k{12};
     return k;
 }
 WC_H int group_kernel() noexcept { return group_kernel_setting().load(std::memory_order_relaxed); }
 WC_H void set_group_kernel(int k) noexcept { group_kernel_setting().store(k, std::memory_order_relaxed); }
 WC_H bool lockstep_inserts() noexcept { return (group_kernel() & 1) != 0; }
 WC_H bool lockstep_lookups() noexcept { return (group_kernel() & 2) != 0; }
+WC_H bool refill_lookups() noexcept { return (group_kernel() & 4) != 0; }
+WC_H bool refill_inserts() noexcept { return (group_kernel() & 8) != 0; }
+
+template<class Core, class StatusHandler, int U>
+WC_H void launch_erase_lockstep(const typename Core::key_type* keys_in, index_t num_in, const Core& core,
+                                index_t probing_length, typename StatusHandler::base_type* status_out,
+                                hipStream_t stream) noexcept {
+    constexpr unsigned CG = unsigned(Core::cg_size());
+    const unsigned grid = grid_for_lockstep<CG, U>(num_in);
+    if (core.geometry().windows < (index_t(1) << 31))
+        kernels::erase_lockstep<Core, StatusHandler, U, std::uint32_t>
+            <<<grid, block_size(), 0, stream>>>(keys_in, num_in, core, probing_length, status_out);
+    else
+        kernels::erase_lockstep<Core, StatusHandler, U, index_t>
+            <<<grid, block_size(), 0, stream>>>(keys_in, num_in, core, probing_length, status_out);
+}
 
 template<class Core, class StatusHandler, int U>
 WC_H void launch_retrieve_lockstep(const typename Core::key_type* keys_in, index_t num_in,

@@ -1,0 +1,483 @@
+// Refill probing for lookups: a group of CG lanes keeps U keys in flight and
+// walks their windows one round at a time, like the lockstep kernel, but a
+// key that finishes is replaced by the group's next key at the start of the
+// following round, so one long probe sequence (a miss at load factor 0.9
+// can walk a dozen windows) no longer idles the other slots of its group or
+// the other groups of its wavefront.
+//
+// What is live is kept as one 64-bit lane mask per slot (bit l: the group of
+// lane l has a key in slot u), which the compiler holds in scalar registers:
+// the decisions of a round are scalar mask arithmetic, a lane's bit is read
+// with inverse_ballot (the mask used as the condition of a select), and the
+// "any lane of my group" of a ballot is one AND with the lane's group mask
+// and one 64-bit compare. Per slot and round a wavefront spends two key
+// compares, two such reductions, the window address and the window advance.
+//
+// Keys enter through a per-group LDS stage of CG entries: lane j of the group
+// hashes key `next + j` of the group's range (first window and step), and a
+// slot being refilled reads the lowest untaken entry (an LDS broadcast).
+// Invalid keys are reported at staging and never take a slot.
+//
+// Per key the windows are read in the same order as the one-key protocol
+// (single_value_hash_table.cuh:267-307): a hit ends the key with its value,
+// a window with an EMPTY lane and no hit ends it as missing, the window
+// budget ends it with probing_length_exceeded. Results and statuses are
+// those of the other kernel families.
+#pragma once
+
+#include "batched_probing.hpp"
+#include "engine.hpp"
+#include "tile.hpp"
+
+namespace warpcore {
+namespace detail {
+
+// Slots per group of the refill kernels.
+#ifndef WARPCORE_REFILL_KEYS
+#define WARPCORE_REFILL_KEYS 8
+#endif
+template<unsigned CG> constexpr int refill_keys() noexcept {
+    return CG < WARPCORE_REFILL_KEYS ? int(CG) : WARPCORE_REFILL_KEYS;
+}
+#ifndef WARPCORE_REFILL_INSERT_KEYS
+#define WARPCORE_REFILL_INSERT_KEYS 4
+#endif
+template<unsigned CG> constexpr int refill_insert_keys() noexcept {
+    return CG < WARPCORE_REFILL_INSERT_KEYS ? int(CG) : WARPCORE_REFILL_INSERT_KEYS;
+}
+
+// One staged key: the key, its first window and its window step.
+template<class K, class W>
+struct alignas(16) RefillEntry {
+    K key;
+    W w, step;
+};
+
+// Lookups of keys_in[first .. end) by the calling group (all lanes of the
+// wavefront call this together; each group passes its own range). W: window
+// index type (std::uint32_t below 2^31 windows). `stage` holds CG entries of
+// this group.
+template<class Core, class StatusHandler, int U, class W>
+WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys_in, index_t first, index_t end,
+                                typename Core::out_type* __restrict__ values_out, const Core& core, index_t budget,
+                                typename StatusHandler::base_type* __restrict__ status_out,
+                                RefillEntry<typename Core::key_type, W>* stage, DeferredJoin& join) {
+    using K = typename Core::key_type;
+    using O = typename Core::out_type;
+    using Slot = typename Core::probe_slot_type;
+    using Cursor = WindowCursor<typename Core::probing_scheme_type>;
+    using Entry = RefillEntry<K, W>;
+    constexpr unsigned CG = unsigned(Core::cg_size());
+    static_assert(CG >= 1 && CG <= 64 && (CG & (CG - 1)) == 0, "groups of 1..64 lanes");
+    static_assert(U >= 1 && U <= 16, "slots per group");
+    const unsigned lane = lane_id();
+    const unsigned rank = lane & (CG - 1);
+    const unsigned shift = lane & ~(CG - 1);
+    constexpr std::uint64_t full = tile<CG>::full_mask();
+    const std::uint64_t gm = CG == 64 ? ~std::uint64_t(0) : (full << shift);   // lanes of my group
+    auto ballot = [](bool p) -> std::uint64_t { return __builtin_amdgcn_ballot_w64(p); };
+    auto mine = [](std::uint64_t m) -> bool { return __builtin_amdgcn_inverse_ballot_w64(m); };
+    // lanes of every group that has a lane in m
+    auto spread = [&](std::uint64_t m) -> std::uint64_t { return ballot((m & gm) != 0); };
+
+    const probing_schemes::Geometry& geo = core.geometry();
+    const W p = W(geo.windows);
+    const Slot* __restrict__ lane_slots = core.slot_ptr() + rank;
+    const index_t len = end > first ? end - first : 0;
+    O* __restrict__ out = values_out + first;
+
+    if (!core.ready()) {   // uniform: every key reports not_initialized
+        for (index_t i = rank; i < len; i += CG) StatusHandler::handle(Status::not_initialized(), status_out, first + i);
+        return;
+    }
+    O miss_value{};
+    const Status miss_status = core.missing(miss_value);
+    // groups that saw a miss / ran out of windows (scalar masks, joined per
+    // group at the end: groups of one wavefront may probe different tables)
+    std::uint64_t missed = 0, exhausted = 0;
+
+    K key[U];
+    W w[U], step[U];
+    std::uint32_t pos[U], rnd[U];
+    std::uint64_t live[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        key[u] = K{};
+        w[u] = 0;
+        step[u] = 1;
+        pos[u] = 0;
+        rnd[u] = 0;
+        live[u] = 0;
+    }
+    index_t next = 0;          // keys of the range staged so far (group-uniform)
+    std::uint64_t avail = 0;   // stage entries holding a key not yet taken (group-uniform)
+    std::uint32_t stage_pos = 0;
+
+    while (true) {
+        // stage the next CG keys of the range once the stage is used up
+        if (avail == 0 && next < len) {
+            const bool in = next + rank < len;
+            const index_t i = first + next + (in ? rank : 0);
+            const K k = keys_in[i];
+            const bool valid = in && core.is_valid_key(k);
+            if (valid) {
+                Cursor c;
+                c.begin(k, core.hash_seed(), geo);
+                Entry e;
+                e.key = k;
+                e.w = W(c.w);
+                e.step = W(c.step);
+                stage[rank] = e;
+            }
+            if (in && !valid) {
+                join(Status::invalid_key());
+                StatusHandler::handle(Status::invalid_key(), status_out, i);
+            }
+            const std::uint64_t vb = ballot(valid);
+            avail = CG == 64 ? vb : ((vb >> shift) & full);
+            stage_pos = std::uint32_t(next);
+            next += CG;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        // refill free slots from the stage, lowest entry first
+        std::uint64_t any_live = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool take = !mine(live[u]) && avail != 0;
+            if (take) {
+                const unsigned j = unsigned(ffs64(avail) - 1);
+                avail &= avail - 1;
+                const Entry e = stage[j];
+                key[u] = e.key;
+                w[u] = e.w;
+                step[u] = e.step;
+                pos[u] = stage_pos + j;
+                rnd[u] = 0;
+            }
+            live[u] |= ballot(take);
+            any_live |= live[u];
+        }
+        if (any_live == 0 && ballot(avail != 0 || next < len) == 0) break;
+        // staging reads and LDS reads of this round are done before the
+        // stage can be overwritten at the top of the next round
+        __builtin_amdgcn_wave_barrier();
+
+        Slot s[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {   // a free slot reads window 0: one line every free slot shares
+            const W at = mine(live[u]) ? w[u] : W(0);
+            s[u] = load_slot(lane_slots + index_t(at) * CG);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const K tk = Core::key_of(s[u]);
+            const std::uint64_t hit = ballot(tk == key[u]) & live[u];
+            const std::uint64_t stop = spread(hit | (ballot(Core::is_empty_key(tk)) & live[u]));
+            const std::uint64_t ghit = spread(hit);
+            const std::uint64_t out_of_windows = live[u] & ~stop & ballot(rnd[u] + 1 >= budget);
+            const std::uint64_t missing = stop & ~ghit;
+            if (mine(hit)) out[pos[u]] = core.found(Core::value_of(s[u]));
+            missed |= missing;
+            exhausted |= out_of_windows;
+            if (rank == 0 && mine(stop | out_of_windows)) {
+                const Status st = mine(ghit) ? Status::none()
+                                             : (mine(missing) ? miss_status : Status::probing_length_exceeded());
+                StatusHandler::handle(st, status_out, first + pos[u]);
+                if (mine(missing) && !miss_status.has_any()) out[pos[u]] = miss_value;
+            }
+            const std::uint64_t cont = live[u] & ~stop & ~out_of_windows;
+            const W nw = w[u] + step[u];
+            w[u] = mine(cont) ? (nw >= p ? W(nw - p) : nw) : w[u];
+            rnd[u] += 1;
+            live[u] = cont;
+        }
+    }
+    if (mine(missed)) join(miss_status);
+    if (mine(exhausted)) join(Status::probing_length_exceeded());
+}
+
+// One staged key of an insert: the key, its first window, its step and the
+// value (or increment) that goes with it.
+template<class K, class W, class V>
+struct alignas(16) RefillInsertEntry {
+    K key;
+    W w, step;
+    V value;
+};
+
+// Inserts of keys_in[first .. end) by the calling group, with U keys in
+// flight (single_value_hash_table.cuh:742-823 per key). A slot is in one of
+// two phases: loading (read the key's current window) or claiming (its
+// window had EMPTY lanes; the lowest untried one issues the CAS). Each
+// round issues the loads of loading slots and the CASes of claiming slots,
+// then moves every slot on; a finished slot writes its result (value store
+// or counter update through finish_insert, then the status) and is
+// refilled the next round. `Core` provides the hooks of insert_batched.
+template<class Core, class StatusHandler, int U, class W, bool HasValues>
+WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_in,
+                              const typename Core::value_type* __restrict__ values_in, index_t first, index_t end,
+                              const Core& core, index_t budget,
+                              typename StatusHandler::base_type* __restrict__ status_out,
+                              RefillInsertEntry<typename Core::key_type, W, typename Core::value_type>* stage,
+                              DeferredJoin& join) {
+    using K = typename Core::key_type;
+    using V = typename Core::value_type;
+    using Probe = typename Core::insert_probe_type;
+    using Cursor = WindowCursor<typename Core::probing_scheme_type>;
+    using Entry = RefillInsertEntry<K, W, V>;
+    constexpr unsigned CG = unsigned(Core::cg_size());
+    static_assert(CG >= 1 && CG <= 64 && (CG & (CG - 1)) == 0, "groups of 1..64 lanes");
+    static_assert(U >= 1 && U <= 16, "slots per group");
+    const unsigned lane = lane_id();
+    const unsigned rank = lane & (CG - 1);
+    const unsigned shift = lane & ~(CG - 1);
+    constexpr std::uint64_t full = tile<CG>::full_mask();
+    const std::uint64_t gm = CG == 64 ? ~std::uint64_t(0) : (full << shift);
+    auto ballot = [](bool p) -> std::uint64_t { return __builtin_amdgcn_ballot_w64(p); };
+    auto mine = [](std::uint64_t m) -> bool { return __builtin_amdgcn_inverse_ballot_w64(m); };
+    auto spread = [&](std::uint64_t m) -> std::uint64_t { return ballot((m & gm) != 0); };
+    auto group_bits = [&](std::uint64_t m) -> std::uint64_t { return CG == 64 ? m : ((m >> shift) & full); };
+
+    const probing_schemes::Geometry& geo = core.geometry();
+    const W p = W(geo.windows);
+    auto* __restrict__ slots = core.slot_ptr();
+    const index_t len = end > first ? end - first : 0;
+
+    if (!core.ready()) {
+        for (index_t i = rank; i < len; i += CG) StatusHandler::handle(Status::not_initialized(), status_out, first + i);
+        return;
+    }
+
+    K key[U];
+    V val[U];
+    W w[U], step[U];
+    std::uint32_t pos[U], rnd[U];
+    std::uint64_t emp[U];   // group bits: EMPTY lanes of the current window not tried yet
+    Probe seen[U];          // this lane's slot as read in the current window
+    std::uint64_t live[U], claim[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        key[u] = K{};
+        val[u] = V{};
+        w[u] = 0;
+        step[u] = 1;
+        pos[u] = 0;
+        rnd[u] = 0;
+        emp[u] = 0;
+        seen[u] = Probe{};
+        live[u] = 0;
+        claim[u] = 0;
+    }
+    std::uint64_t dups = 0, exhausted = 0;   // groups that saw a duplicate / ran out of windows
+    index_t next = 0;
+    std::uint64_t avail = 0;
+    std::uint32_t stage_pos = 0;
+
+    while (true) {
+        if (avail == 0 && next < len) {
+            const bool in = next + rank < len;
+            const index_t i = first + next + (in ? rank : 0);
+            const K k = keys_in[i];
+            const bool valid = in && core.is_valid_key(k);
+            if (valid) {
+                Cursor c;
+                c.begin(k, core.hash_seed(), geo);
+                Entry e;
+                e.key = k;
+                e.w = W(c.w);
+                e.step = W(c.step);
+                if constexpr (HasValues) e.value = values_in[i];
+                else e.value = V{};
+                stage[rank] = e;
+            }
+            if (in && !valid) {
+                join(Status::invalid_key());
+                StatusHandler::handle(Status::invalid_key(), status_out, i);
+            }
+            avail = group_bits(ballot(valid));
+            stage_pos = std::uint32_t(next);
+            next += CG;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
+        std::uint64_t any_live = 0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool take = !mine(live[u]) && avail != 0;
+            if (take) {
+                const unsigned j = unsigned(ffs64(avail) - 1);
+                avail &= avail - 1;
+                const Entry e = stage[j];
+                key[u] = e.key;
+                w[u] = e.w;
+                step[u] = e.step;
+                val[u] = e.value;
+                pos[u] = stage_pos + j;
+                rnd[u] = 0;
+            }
+            const std::uint64_t t = ballot(take);
+            live[u] |= t;
+            claim[u] &= ~t;
+            any_live |= live[u];
+        }
+        if (any_live == 0 && ballot(avail != 0 || next < len) == 0) break;
+        __builtin_amdgcn_wave_barrier();
+
+        // issue: window loads of loading slots, claiming CASes
+        Probe fresh[U];
+        typename Core::cas_word old[U];
+        bool lead[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {   // a slot not loading reads window 0: one line it shares
+            const W at = mine(live[u] & ~claim[u]) ? w[u] : W(0);
+            fresh[u] = Core::load_insert_probe(slots + index_t(at) * CG + rank);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            lead[u] = mine(claim[u]) && rank == unsigned(ffs64(emp[u]) - 1);
+            old[u] = typename Core::cas_word{};
+            if (lead[u]) old[u] = core.cas_issue(slots + index_t(w[u]) * CG + rank, seen[u], key[u], val[u]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const std::uint64_t loading = live[u] & ~claim[u];
+            const std::uint64_t claiming = live[u] & claim[u];
+            // loading slots: a hit is a duplicate, EMPTY lanes start the claim
+            const K tk = Core::probe_key(fresh[u]);
+            const std::uint64_t hitl = ballot(tk == key[u]) & loading;
+            const std::uint64_t ghit = spread(hitl);
+            const std::uint64_t gemp = spread(ballot(Core::is_empty_key(tk)) & loading) & ~ghit;
+            if (mine(loading)) seen[u] = fresh[u];
+            if (mine(gemp)) emp[u] = group_bits(ballot(Core::is_empty_key(tk)));
+            // claiming slots: the leader's CAS outcome
+            const int outcome = lead[u] ? core.cas_outcome(old[u], seen[u], key[u]) : -1;
+            const std::uint64_t won = spread(ballot(outcome == 1)) & claiming;
+            const std::uint64_t dupc = spread(ballot(outcome == 2)) & claiming;
+            const std::uint64_t retry = spread(ballot(outcome == 3)) & claiming;
+            const std::uint64_t lost = claiming & ~won & ~dupc & ~retry;
+            const unsigned leader = unsigned(ffs64(emp[u]) - 1);
+            if (mine(lost)) emp[u] &= emp[u] - 1;
+            const std::uint64_t drained = lost & ballot(emp[u] == 0);
+            // finished keys: duplicate (found in the window or returned by
+            // the CAS) or claimed
+            const std::uint64_t dup = ghit | dupc;
+            const std::uint64_t done = dup | won;
+            dups |= dup;
+            if (rank == 0 && mine(done)) {
+                const unsigned at_lane = mine(ghit) ? unsigned(ffs64(group_bits(hitl)) - 1) : leader;
+                Status st = mine(dup) ? Status::duplicate_key() : Status::none();
+                core.finish_insert(index_t(w[u]) * CG + at_lane, val[u], mine(won) && Core::publishes_value, st,
+                                   join);
+                StatusHandler::handle(st, status_out, first + pos[u]);
+            }
+            // slots moving to the next window, or out of windows
+            const std::uint64_t adv = (loading & ~ghit & ~gemp) | drained;
+            const std::uint64_t last = adv & ballot(rnd[u] + 1 >= budget);
+            exhausted |= last;
+            if (rank == 0 && mine(last)) StatusHandler::handle(Status::probing_length_exceeded(), status_out, first + pos[u]);
+            const std::uint64_t move = adv & ~last;
+            const W nw = w[u] + step[u];
+            if (mine(move)) {
+                w[u] = nw >= p ? W(nw - p) : nw;
+                rnd[u] += 1;
+            }
+            claim[u] = (claim[u] | gemp) & ~(done | adv);
+            live[u] &= ~(done | last);
+        }
+    }
+    if (mine(dups)) join(Status::duplicate_key());
+    if (mine(exhausted)) join(Status::probing_length_exceeded());
+}
+
+}  // namespace detail
+
+namespace kernels {
+
+// Wavefronts per SIMD the refill lookup kernel is compiled for (its
+// register budget: 512 / waves VGPRs); 0 leaves the choice to the compiler.
+// At 4 (128 VGPRs) the u64 kernel runs 4 wavefronts per SIMD instead of 3
+// (141 VGPRs), which is worth 13 % on a 4.8 GB table.
+#ifndef WARPCORE_REFILL_WAVES
+#define WARPCORE_REFILL_WAVES 4
+#endif
+#if WARPCORE_REFILL_WAVES > 0
+#define WC_REFILL_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WARPCORE_REFILL_WAVES)))
+#else
+#define WC_REFILL_OCCUPANCY
+#endif
+
+// Bulk lookups: group g of the grid takes keys [g * per, (g + 1) * per).
+template<class Core, class StatusHandler, int U, class W>
+__global__ __launch_bounds__(256) WC_REFILL_OCCUPANCY void retrieve_refill(const typename Core::key_type* __restrict__ keys_in,
+                                                       index_t num_in, typename Core::out_type* __restrict__ values_out,
+                                                       Core core, index_t probing_length,
+                                                       typename StatusHandler::base_type* __restrict__ status_out) {
+    constexpr unsigned CG = unsigned(Core::cg_size());
+    __shared__ detail::RefillEntry<typename Core::key_type, W> stage[256];
+    detail::DeferredJoin join;
+    const index_t gid = (index_t(blockIdx.x) * blockDim.x + threadIdx.x) / CG;
+    const index_t groups = (index_t(gridDim.x) * blockDim.x) / CG;
+    const index_t per = SDIV(num_in, groups);
+    const index_t first = gid * per < num_in ? gid * per : num_in;
+    const index_t end = first + per < num_in ? first + per : num_in;
+    const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, core.geometry().windows);
+    detail::retrieve_refill_range<Core, StatusHandler, U, W>(keys_in, first, end, values_out, core, budget,
+                                                             status_out, stage + (threadIdx.x & ~(CG - 1)), join);
+    join.flush(core.status_word());
+}
+
+template<class Core, class StatusHandler, int U, class W, bool HasValues>
+__global__ __launch_bounds__(256) void insert_refill(const typename Core::key_type* __restrict__ keys_in,
+                                                     const typename Core::value_type* __restrict__ values_in,
+                                                     index_t num_in, Core core, index_t probing_length,
+                                                     typename StatusHandler::base_type* __restrict__ status_out) {
+    constexpr unsigned CG = unsigned(Core::cg_size());
+    __shared__ detail::RefillInsertEntry<typename Core::key_type, W, typename Core::value_type> stage[256];
+    detail::DeferredJoin join;
+    const index_t gid = (index_t(blockIdx.x) * blockDim.x + threadIdx.x) / CG;
+    const index_t groups = (index_t(gridDim.x) * blockDim.x) / CG;
+    const index_t per = SDIV(num_in, groups);
+    const index_t first = gid * per < num_in ? gid * per : num_in;
+    const index_t end = first + per < num_in ? first + per : num_in;
+    const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, core.geometry().windows);
+    detail::insert_refill_range<Core, StatusHandler, U, W, HasValues>(keys_in, values_in, first, end, core, budget,
+                                                                      status_out, stage + (threadIdx.x & ~(CG - 1)),
+                                                                      join);
+    join.flush(core.status_word());
+}
+
+}  // namespace kernels
+
+namespace detail {
+
+template<class Core, class StatusHandler, int U, bool HasValues>
+WC_H void launch_insert_refill(const typename Core::key_type* keys_in, const typename Core::value_type* values_in,
+                               index_t num_in, const Core& core, index_t probing_length,
+                               typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
+    const unsigned grid = unsigned(num_compute_units() * 8);
+    if (core.geometry().windows < (index_t(1) << 31))
+        kernels::insert_refill<Core, StatusHandler, U, std::uint32_t, HasValues>
+            <<<grid, block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out);
+    else
+        kernels::insert_refill<Core, StatusHandler, U, index_t, HasValues>
+            <<<grid, block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out);
+}
+
+template<class Core, class StatusHandler, int U>
+WC_H void launch_retrieve_refill(const typename Core::key_type* keys_in, index_t num_in,
+                                 typename Core::out_type* values_out, const Core& core, index_t probing_length,
+                                 typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
+    const unsigned grid = unsigned(num_compute_units() * 8);
+    if (core.geometry().windows < (index_t(1) << 31))
+        kernels::retrieve_refill<Core, StatusHandler, U, std::uint32_t>
+            <<<grid, block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
+    else
+        kernels::retrieve_refill<Core, StatusHandler, U, index_t>
+            <<<grid, block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
+}
+
+}  // namespace detail
+}  // namespace warpcore

@@ -3,34 +3,41 @@
 //
 // No reference counterpart (the reference is a library of one-call-per-
 // batch operations). Clients submit requests (device buffers of keys,
-// values, outputs); a scheduler thread admits them, coalesces every
-// admitted request into one step, and runs a step as one fixed-shape
-// sequence of launches (descriptor upload, work-item listing, one
-// persistent kernel over (request, chunk) items), captured once as a HIP
-// graph and replayed. Steps are issued back to back on one stream while
-// the host already builds the next one, so requests that arrive while a
-// step runs join the following step; a request completes when the event
-// recorded after its step has fired.
+// values, outputs); a scheduler thread admits them, coalesces the admitted
+// requests into steps, and runs a step as one fixed-shape sequence of
+// launches (work-item listing and one persistent kernel over small work
+// items of every request in the step), captured once as a HIP graph and
+// replayed; only the request descriptors are uploaded per step. Steps are
+// issued back to back on one stream while the host already builds the next
+// one, so requests that arrive while a step runs join the following step; a
+// request completes when the event recorded after its (last) step has fired.
+//
+// A step holds at most max_step_keys keys and max_requests requests: a
+// request larger than what is left of a step runs in chunks over several
+// steps, so a large request does not hold up the small ones that arrive
+// behind it for longer than one bounded step.
 //
 // Admission against the HBM budget: every table has a slot budget of
 // max_load * capacity. An insert of n keys reserves n slots of its table
 // (a key may be a duplicate, so this is an upper bound); slots are never
 // given back, because erase leaves tombstones that keep their slots, so an
 // insert that does not fit its table's remaining budget is refused (status
-// out_of_memory for every key). A step holds at most max_requests requests
-// and max_items work items; what does not fit waits for the next step.
+// out_of_memory for every key).
 //
 // Order: per table, requests run in arrival order. A step takes, per
-// table, either any run of consecutive lookups, or one insert or erase, so
-// no two requests of one step conflict; steps run in order on the stream.
-// Each request therefore sees exactly the table state it would see if all
-// requests of its table ran one at a time in arrival order, which is what
-// the tests replay.
+// table, either any run of consecutive lookups, or one insert or erase (or
+// one chunk of it), so no two requests of one step conflict; steps run in
+// order on the stream. Each request therefore sees exactly the table state
+// it would see if all requests of its table ran one at a time in arrival
+// order, which is what the tests replay. (An insert that runs in chunks
+// behaves like the same keys inserted by consecutive calls: for duplicate
+// keys across chunks the first chunk's copy reports `none`.)
 #pragma once
 
 #include "base.hpp"
 #include "engine.hpp"
 #include "lockstep_probing.hpp"
+#include "refill_probing.hpp"
 #include "scan.hpp"
 #include "single_value_hash_table.hpp"
 #include "status.hpp"
@@ -74,56 +81,50 @@ __global__ __launch_bounds__(256) void serve_count_items(const Desc* __restrict_
     if (r < max_reqs) items[r] = r < *num_reqs ? SDIV(reqs[r].n, item_keys) : 0;
 }
 
+// Work-item listing: item i (of first[R-1] + items[R-1]) belongs to the last
+// request r with first[r] <= i (first is the exclusive scan of items).
 template<class T>
-__global__ __launch_bounds__(256) void serve_fill_items(const index_t* __restrict__ first,
-                                                        const index_t* __restrict__ items, index_t max_reqs,
-                                                        std::uint32_t* __restrict__ item_req,
-                                                        index_t* __restrict__ total, index_t* __restrict__ counter) {
-    const index_t r = index_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r < max_reqs)
-        for (index_t j = 0; j < items[r]; ++j) item_req[first[r] + j] = std::uint32_t(r);
-    if (r == 0) {
-        *total = first[max_reqs - 1] + items[max_reqs - 1];
-        *counter = 0;
+__global__ __launch_bounds__(256) void serve_fill_items(const T* __restrict__ first, const T* __restrict__ items,
+                                                        index_t max_reqs, std::uint32_t* __restrict__ item_req) {
+    const T total = first[max_reqs - 1] + items[max_reqs - 1];
+    const index_t stride = index_t(gridDim.x) * blockDim.x;
+    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
+        index_t lo = 0, hi = max_reqs;   // upper_bound(first, i) - 1
+        while (hi - lo > 1) {
+            const index_t mid = (lo + hi) / 2;
+            if (first[mid] <= T(i)) lo = mid;
+            else hi = mid;
+        }
+        item_req[i] = std::uint32_t(lo);
     }
 }
 
-// Erase keys_in[first], keys_in[first + 1], ... below end, one key per
-// group at a time (single_value_hash_table.cuh:425-468).
-template<class View, class StatusHandler>
-WC_D void serve_erase_range(const typename View::key_type* keys_in, index_t first, index_t end, const View& view,
-                            index_t probing_length, typename StatusHandler::base_type* status_out,
-                            detail::DeferredJoin& join) {
-    constexpr unsigned CG = unsigned(View::cg_size());
-    const tile<CG> group;
-    for (index_t i = first; i < end; ++i) {
-        const Status st = view.erase(keys_in[i], group, probing_length, join);
-        if (group.thread_rank() == 0) StatusHandler::handle(st, status_out, i);
-    }
-}
-
-template<class View, int UR, int UI>
+// One step: group g of the grid takes items g, g + G, ... (items hold
+// item_keys keys each, so a static assignment balances). Lookups and
+// inserts use the refill kernels' range code, erases the lockstep one.
+template<class View, int UR, int UI, int UE>
 __global__ __launch_bounds__(256) void serve_step(const serving::RequestDesc<typename View::key_type,
                                                                              typename View::value_type>* __restrict__ reqs,
                                                   const std::uint32_t* __restrict__ item_req,
                                                   const index_t* __restrict__ first_item,
-                                                  const index_t* __restrict__ total_items,
-                                                  index_t* __restrict__ counter, index_t item_keys,
-                                                  const View* __restrict__ views, index_t probing_length) {
+                                                  const index_t* __restrict__ items, index_t max_reqs,
+                                                  index_t item_keys, const View* __restrict__ views,
+                                                  index_t probing_length) {
     using K = typename View::key_type;
     using W = std::uint32_t;
     constexpr unsigned CG = unsigned(View::cg_size());
-    constexpr int UMAX = UR > UI ? UR : UI;
-    __shared__ detail::StagedKey<K, W> stage[256 / CG * UMAX];
-    auto* my = stage + (threadIdx.x / CG) * UMAX;
+    __shared__ detail::StagedKey<K, W> stage[256 / CG * UE];
+    __shared__ detail::RefillEntry<K, W> refill[256];
+    __shared__ detail::RefillInsertEntry<K, W, typename View::value_type> refill_insert[256];
+    auto* my = stage + (threadIdx.x / CG) * UE;
+    auto* my_refill = refill + (threadIdx.x & ~(CG - 1));
+    auto* my_refill_insert = refill_insert + (threadIdx.x & ~(CG - 1));
     const unsigned rank = lane_id() & (CG - 1);
-    const index_t total = *total_items;
+    const index_t total = first_item[max_reqs - 1] + items[max_reqs - 1];
+    const index_t gid = (index_t(blockIdx.x) * blockDim.x + threadIdx.x) / CG;
+    const index_t groups = (index_t(gridDim.x) * blockDim.x) / CG;
     using H = status_handlers::ReturnStatus;
-    while (true) {
-        index_t item = 0;
-        if (rank == 0) item = detail::atomic_add(counter, index_t(1));
-        item = tile<CG>::shfl(item, 0);
-        if (item >= total) break;
+    for (index_t item = gid; item < total; item += groups) {
         const std::uint32_t r = item_req[item];
         const auto d = reqs[r];
         const index_t begin = (item - first_item[r]) * item_keys;
@@ -133,13 +134,13 @@ __global__ __launch_bounds__(256) void serve_step(const serving::RequestDesc<typ
         detail::DeferredJoin join;
         auto* st = reinterpret_cast<Status*>(d.status_out);
         if (d.op == std::uint32_t(serving::Op::insert))
-            detail::insert_lockstep_range<View, H, UI, W, true>(d.keys, d.values, begin, end, UI, view, budget, st,
-                                                                my, join);
+            detail::insert_refill_range<View, H, UI, W, true>(d.keys, d.values, begin, end, view, budget, st,
+                                                              my_refill_insert, join);
         else if (d.op == std::uint32_t(serving::Op::retrieve))
-            detail::retrieve_lockstep_range<View, H, UR, W>(d.keys, begin, end, UR, d.values_out, view, budget, st,
-                                                            my, join);
+            detail::retrieve_refill_range<View, H, UR, W>(d.keys, begin, end, d.values_out, view, budget, st,
+                                                          my_refill, join);
         else
-            serve_erase_range<View, H>(d.keys, begin, end, view, probing_length, st, join);
+            detail::erase_lockstep_range<View, H, UE, W>(d.keys, begin, end, UE, view, budget, st, my, join);
         const status_base_t all = tile<CG>::reduce_or(join.acc);
         if (rank == 0 && all != 0) view.status->atomic_join(Status::from_base(all));
     }
@@ -164,23 +165,29 @@ public:
     using clock = std::chrono::steady_clock;
 
     struct Config {
-        index_t max_requests = 1024;      // requests per step (fixed shape)
-        index_t item_keys = 1024;         // keys per work item
-        float max_load = 0.9f;            // admission: slot budget per table
-        int blocks_per_cu = 8;            // persistent grid
-        bool use_graph = true;            // replay a captured step
+        index_t max_requests = 1024;                 // requests (chunks) per step: descriptor array
+        index_t item_keys = 64;                      // keys per work item
+        index_t max_step_keys = index_t(1) << 24;    // keys per step
+        float max_load = 0.9f;                       // admission: slot budget per table
+        int blocks_per_cu = 8;                       // persistent grid
+        bool use_graph = true;                       // replay a captured step
     };
 
     struct Request {
         desc_type d;
         index_t id;
+        index_t issued = 0;              // keys handed to steps so far
         clock::time_point arrival, done;
-        std::uint64_t step = 0;          // step that ran it (0 = not yet)
+        std::uint64_t step = 0;          // step that finished it (0 = not yet)
         std::atomic<int> state{0};       // 0 queued, 1 running, 2 done
         bool refused = false;
     };
 
     Server(const index_t* table_capacities, index_t num_tables, Config cfg = Config{}) : cfg_(cfg) {
+        if (cfg_.item_keys == 0 || cfg_.max_requests == 0 || cfg_.max_step_keys == 0) {
+            ok_ = false;
+            return;
+        }
         for (index_t t = 0; t < num_tables; ++t) {
             tables_.emplace_back(new table_type(table_capacities[t]));
             // the step kernel keeps window indices in 32 bits
@@ -191,11 +198,11 @@ public:
             budget_.push_back(index_t(double(tables_.back()->capacity()) * cfg_.max_load));
             reserved_.push_back(0);
         }
+        max_items_ = SDIV(cfg_.max_step_keys, cfg_.item_keys) + cfg_.max_requests;
         std::vector<view_type> views;
         for (auto& t : tables_) views.push_back(t->view());
-        ok_ = alloc(&views_d_, views.size()) && alloc(&item_req_, cfg_.max_requests * 64 + 1024) &&
-              alloc(&items_, cfg_.max_requests) && alloc(&first_, cfg_.max_requests) && alloc(&total_, 1) &&
-              alloc(&counter_, 1) && alloc(&parts_, detail::kScanParts) && alloc(&num_reqs_d_, 1);
+        ok_ = alloc(&views_d_, views.size()) && alloc(&item_req_, max_items_) && alloc(&items_, cfg_.max_requests) &&
+              alloc(&first_, cfg_.max_requests) && alloc(&parts_, detail::kScanParts) && alloc(&num_reqs_d_, 1);
         for (int b = 0; b < 2 && ok_; ++b) {
             ok_ = alloc(&desc_d_[b], cfg_.max_requests) &&
                   hipHostMalloc(reinterpret_cast<void**>(&desc_h_[b]), sizeof(desc_type) * cfg_.max_requests) ==
@@ -206,6 +213,7 @@ public:
         // blocking streams: work the client queued on the default stream
         // (filling its buffers) is ordered before the steps
         ok_ = hipStreamCreate(&stream_) == hipSuccess && hipStreamCreate(&side_) == hipSuccess &&
+              hipStreamCreateWithFlags(&capture_, hipStreamNonBlocking) == hipSuccess &&
               hipMemcpy(views_d_, views.data(), sizeof(view_type) * views.size(), hipMemcpyHostToDevice) == hipSuccess;
         if (!ok_) return;
         worker_ = std::thread([this] { loop(); });
@@ -219,8 +227,8 @@ public:
         if (worker_.joinable()) worker_.join();
         if (stream_) (void)hipStreamSynchronize(stream_);
         for (auto& g : graphs_) if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        for (void* p : {(void*)views_d_, (void*)item_req_, (void*)items_, (void*)first_, (void*)total_,
-                        (void*)counter_, (void*)parts_, (void*)num_reqs_d_, (void*)desc_d_[0], (void*)desc_d_[1]})
+        for (void* p : {(void*)views_d_, (void*)item_req_, (void*)items_, (void*)first_, (void*)parts_,
+                        (void*)num_reqs_d_, (void*)desc_d_[0], (void*)desc_d_[1]})
             if (p) (void)hipFree(p);
         for (int b = 0; b < 2; ++b) {
             if (desc_h_[b]) (void)hipHostFree(desc_h_[b]);
@@ -228,6 +236,7 @@ public:
         }
         if (stream_) (void)hipStreamDestroy(stream_);
         if (side_) (void)hipStreamDestroy(side_);
+        if (capture_) (void)hipStreamDestroy(capture_);
     }
 
     bool ok() const noexcept { return ok_; }
@@ -291,6 +300,12 @@ public:
     }
 
 private:
+    struct Chunk {
+        std::shared_ptr<Request> r;
+        bool last;   // the request's final chunk
+    };
+    using StepList = std::vector<Chunk>;
+
     template<class T>
     static bool alloc(T** p, index_t n) {
         *p = nullptr;
@@ -311,40 +326,50 @@ private:
         done_cv_.notify_all();
     }
 
-    // Requests of the next step, chosen under mu_: per table in arrival
-    // order, a run of lookups or one mutation; inserts must fit the budget.
-    void pick(std::vector<std::shared_ptr<Request>>& step) {
+    // Requests (chunks) of the next step, chosen under mu_: per table in
+    // arrival order, a run of lookups or one mutation; an insert is admitted
+    // against its table's budget when its first chunk is taken. Writes the
+    // chunk descriptors to `descs`.
+    void pick(StepList& step, desc_type* descs) {
         std::vector<int> mode(tables_.size(), -1);   // -1 untouched, 0 lookups, 1 mutation, 2 closed
         std::deque<std::shared_ptr<Request>> keep;
-        index_t items = 0;
-        const index_t max_items = cfg_.max_requests * 64 + 1024;
+        index_t keys = 0;
         while (!pending_.empty()) {
             auto r = pending_.front();
             pending_.pop_front();
             const std::uint32_t t = r->d.table;
             const Op op = Op(r->d.op);
-            const index_t it = SDIV(r->d.n, cfg_.item_keys);
-            bool take = step.size() < cfg_.max_requests && items + it <= max_items;
-            if (take) {
-                if (mode[t] == 2 || mode[t] == 1) take = false;
-                else if (op == Op::retrieve) take = true;
-                else take = mode[t] == -1;
+            bool take = step.size() < cfg_.max_requests && keys < cfg_.max_step_keys && mode[t] != 2 &&
+                        mode[t] != 1 && (op == Op::retrieve || mode[t] == -1);
+            if (take && op == Op::insert && r->issued == 0) {
+                if (reserved_[t] + r->d.n > budget_[t]) {
+                    finish_refused(*r);
+                    continue;
+                }
+                reserved_[t] += r->d.n;
             }
-            if (take && op == Op::insert && reserved_[t] + r->d.n > budget_[t]) {
-                finish_refused(*r);
-                continue;
-            }
-            if (it > max_items) {   // larger than any step
-                finish_refused(*r);
-                continue;
-            }
-            if (take) {
-                if (op == Op::insert) reserved_[t] += r->d.n;
-                mode[t] = op == Op::retrieve ? 0 : 1;
-                items += it;
-                step.push_back(r);
-            } else {
+            if (!take) {
                 mode[t] = 2;   // later requests of this table wait behind it
+                keep.push_back(r);
+                continue;
+            }
+            const index_t left = r->d.n - r->issued;
+            const index_t room = cfg_.max_step_keys - keys;
+            const index_t c = left < room ? left : room;
+            desc_type d = r->d;
+            d.keys += r->issued;
+            if (d.values) d.values += r->issued;
+            if (d.values_out) d.values_out += r->issued;
+            if (d.status_out) d.status_out += r->issued;
+            d.n = c;
+            descs[step.size()] = d;
+            r->issued += c;
+            const bool last = r->issued == r->d.n;
+            step.push_back(Chunk{r, last});
+            keys += c;
+            mode[t] = op == Op::retrieve ? 0 : 1;
+            if (!last) {   // the rest runs in a later step, ahead of this table's later requests
+                mode[t] = 2;
                 keep.push_back(r);
             }
         }
@@ -356,99 +381,105 @@ private:
     };
 
     // Work-item listing plus the persistent kernel of one step, reading the
-    // descriptors at desc_d_[b] (fixed shape: max_requests descriptors).
-    void enqueue_step(int b) {
+    // descriptors at desc_d_[b] (fixed shape: max_requests descriptors, of
+    // which the first *num_reqs_d_ are live).
+    void enqueue_step(int b, hipStream_t stream) {
         const index_t R = cfg_.max_requests;
         const unsigned rg = unsigned(SDIV(R, index_t(256)));
-        kernels::serve_count_items<desc_type><<<rg, 256, 0, stream_>>>(desc_d_[b], num_reqs_d_, R, cfg_.item_keys, items_);
-        detail::device_scan(items_, first_, R, parts_, false, stream_);
-        kernels::serve_fill_items<index_t><<<rg, 256, 0, stream_>>>(first_, items_, R, item_req_, total_, counter_);
-        const unsigned grid = unsigned(detail::num_compute_units() * cfg_.blocks_per_cu);
-        kernels::serve_step<view_type, detail::lockstep_keys<unsigned(CG)>(),
-                            detail::lockstep_insert_keys<unsigned(CG)>()>
-            <<<grid, block_size(), 0, stream_>>>(desc_d_[b], item_req_, first_, total_, counter_, cfg_.item_keys,
-                                                 views_d_, defaults::probing_length());
+        const unsigned cus = unsigned(detail::num_compute_units());
+        kernels::serve_count_items<desc_type><<<rg, 256, 0, stream>>>(desc_d_[b], num_reqs_d_, R, cfg_.item_keys, items_);
+        detail::device_scan(items_, first_, R, parts_, false, stream);
+        kernels::serve_fill_items<index_t><<<cus * 4, 256, 0, stream>>>(first_, items_, R, item_req_);
+        const unsigned grid = unsigned(cus * cfg_.blocks_per_cu);
+        kernels::serve_step<view_type, detail::refill_keys<unsigned(CG)>(),
+                            detail::refill_insert_keys<unsigned(CG)>(), detail::lockstep_keys<unsigned(CG)>()>
+            <<<grid, block_size(), 0, stream>>>(desc_d_[b], item_req_, first_, items_, R, cfg_.item_keys, views_d_,
+                                                 defaults::probing_length());
     }
 
-    void launch(int b) {
-        // the request count and descriptors travel with the step
+    void launch(int b, index_t count) {
+        // the request count and the live descriptors travel with the step
         (void)hipMemcpyAsync(num_reqs_d_, num_reqs_h_[b], sizeof(std::uint32_t), hipMemcpyHostToDevice, stream_);
-        (void)hipMemcpyAsync(desc_d_[b], desc_h_[b], sizeof(desc_type) * cfg_.max_requests, hipMemcpyHostToDevice,
-                             stream_);
+        (void)hipMemcpyAsync(desc_d_[b], desc_h_[b], sizeof(desc_type) * count, hipMemcpyHostToDevice, stream_);
         if (!cfg_.use_graph) {
-            enqueue_step(b);
+            enqueue_step(b, stream_);
             return;
         }
         if (!graphs_[b].exec) {
+            // captured on a non-blocking stream that never runs work, so
+            // that clients using the legacy default stream meanwhile do not
+            // collide with a capture in progress
             hipGraph_t g = nullptr;
-            bool ok = hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            bool ok = hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal) == hipSuccess;
             if (ok) {
-                enqueue_step(b);
-                ok = hipStreamEndCapture(stream_, &g) == hipSuccess &&
+                enqueue_step(b, capture_);
+                ok = hipStreamEndCapture(capture_, &g) == hipSuccess &&
                      hipGraphInstantiate(&graphs_[b].exec, g, nullptr, nullptr, 0) == hipSuccess;
             }
             if (g) (void)hipGraphDestroy(g);
             if (!ok) {
                 graphs_[b].exec = nullptr;
                 cfg_.use_graph = false;
-                enqueue_step(b);
+                enqueue_step(b, stream_);
                 return;
             }
         }
         (void)hipGraphLaunch(graphs_[b].exec, stream_);
     }
 
-    void retire(std::deque<std::pair<hipEvent_t, std::vector<std::shared_ptr<Request>>>>& inflight) {
+    struct InFlight {
+        hipEvent_t ev;
+        std::uint64_t step_no;
+        StepList chunks;
+    };
+
+    void retire(std::deque<InFlight>& inflight) {
         const auto now = clock::now();
         {
             std::lock_guard<std::mutex> g(mu_);
-            for (auto& r : inflight.front().second) {
-                r->done = now;
-                r->state.store(2);
+            for (auto& c : inflight.front().chunks) {
+                if (!c.last) continue;
+                c.r->done = now;
+                c.r->step = inflight.front().step_no;
+                c.r->state.store(2);
                 stats_.requests++;
-                stats_.keys += r->d.n;
+                stats_.keys += c.r->d.n;
             }
         }
         done_cv_.notify_all();
-        (void)hipEventDestroy(inflight.front().first);
+        (void)hipEventDestroy(inflight.front().ev);
         inflight.pop_front();
     }
 
     void loop() {
-        std::deque<std::pair<hipEvent_t, std::vector<std::shared_ptr<Request>>>> inflight;
+        std::deque<InFlight> inflight;
         int b = 0;
         std::uint64_t step_no = 0;
         while (true) {
-            std::vector<std::shared_ptr<Request>> step;
+            StepList step;
             {
                 std::unique_lock<std::mutex> g(mu_);
                 if (inflight.empty()) cv_.wait(g, [&] { return stop_ || !pending_.empty(); });
                 if (stop_ && pending_.empty() && inflight.empty()) break;
                 // at most two steps in flight: descriptors are double-buffered
-                if (inflight.size() < 2) pick(step);
+                if (inflight.size() < 2) pick(step, desc_h_[b]);
+                for (auto& c : step) c.r->state.store(1);
+                if (!step.empty()) stats_.steps++;
             }
             if (!step.empty()) {
                 ++step_no;
                 *num_reqs_h_[b] = std::uint32_t(step.size());
-                for (index_t i = 0; i < cfg_.max_requests; ++i)
-                    desc_h_[b][i] = i < step.size() ? step[i]->d : desc_type{};
-                for (auto& r : step) {
-                    r->step = step_no;
-                    r->state.store(1);
-                }
-                launch(b);
+                launch(b, step.size());
                 hipEvent_t ev = nullptr;
                 (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
                 (void)hipEventRecord(ev, stream_);
-                inflight.emplace_back(ev, std::move(step));
+                inflight.push_back(InFlight{ev, step_no, std::move(step)});
                 b ^= 1;
-                std::lock_guard<std::mutex> g(mu_);
-                stats_.steps++;
             }
             // retire finished steps; with a free descriptor buffer, go back
             // and build the next step as soon as requests are pending
             while (!inflight.empty()) {
-                if (hipEventQuery(inflight.front().first) == hipSuccess) {
+                if (hipEventQuery(inflight.front().ev) == hipSuccess) {
                     retire(inflight);
                     continue;
                 }
@@ -459,7 +490,7 @@ private:
                     if (!pending_.empty() || stop_) break;
                     continue;
                 }
-                (void)hipEventSynchronize(inflight.front().first);
+                (void)hipEventSynchronize(inflight.front().ev);
                 retire(inflight);
             }
         }
@@ -467,6 +498,7 @@ private:
 
     Config cfg_;
     bool ok_ = true;
+    index_t max_items_ = 0;
     std::vector<std::unique_ptr<table_type>> tables_;
     std::vector<index_t> budget_, reserved_;
     view_type* views_d_ = nullptr;
@@ -475,9 +507,10 @@ private:
     std::uint32_t* num_reqs_h_[2] = {nullptr, nullptr};
     std::uint32_t* num_reqs_d_ = nullptr;
     std::uint32_t* item_req_ = nullptr;
-    index_t *items_ = nullptr, *first_ = nullptr, *total_ = nullptr, *counter_ = nullptr, *parts_ = nullptr;
+    index_t *items_ = nullptr, *first_ = nullptr, *parts_ = nullptr;
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;
+    hipStream_t capture_ = nullptr;
     Graph graphs_[2];
     std::mutex mu_;
     std::condition_variable cv_, done_cv_;

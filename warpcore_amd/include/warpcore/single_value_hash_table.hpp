@@ -20,6 +20,7 @@
 #include "batched_probing.hpp"
 #include "lane_probing.hpp"
 #include "lockstep_probing.hpp"
+#include "refill_probing.hpp"
 #include "engine.hpp"
 #include "probing_schemes.hpp"
 #include "status.hpp"
@@ -76,6 +77,8 @@ struct SingleValueView {
     WC_HD static Value value_of(const Slot& s) noexcept { return s.value; }
     WC_D Value found(const Value& v) const noexcept { return v; }
     WC_D Status missing(Value&) const noexcept { return Status::key_not_found(); }
+    // erase of the key found at slot i (single_value_hash_table.cuh:446-455)
+    WC_D void erase_slot(index_type i) const noexcept { slots[i].key = TombstoneKey; }
     // Claiming CAS of the batched insert, split so that the U CASes of a
     // group are all in flight before any result is examined. cas_issue()
     // swaps the EMPTY slot this lane saw as `seen` for the key (packed slots:
@@ -374,6 +377,11 @@ public:
             }
         }
         if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+            if (detail::refill_inserts()) {
+                detail::launch_insert_refill<view_type, StatusHandler, detail::refill_insert_keys<unsigned(cg_size())>(), true>(
+                    keys_in, values_in, num_in, view(), probing_length, status_out, stream);
+                return;
+            }
             if (detail::lockstep_inserts()) {
                 detail::launch_insert_lockstep<view_type, StatusHandler,
                                                detail::lockstep_insert_keys<unsigned(cg_size())>(), true>(
@@ -402,6 +410,11 @@ public:
             }
         }
         if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+            if (detail::refill_lookups()) {
+                detail::launch_retrieve_refill<view_type, StatusHandler, detail::refill_keys<unsigned(cg_size())>()>(
+                    keys_in, num_in, values_out, view(), probing_length, status_out, stream);
+                return;
+            }
             if (detail::lockstep_lookups()) {
                 detail::launch_retrieve_lockstep<view_type, StatusHandler, detail::lockstep_keys<unsigned(cg_size())>()>(
                     keys_in, num_in, values_out, view(), probing_length, status_out, stream);
@@ -432,6 +445,13 @@ public:
                     typename StatusHandler::base_type* status_out = nullptr) noexcept {
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!is_initialized_ || num_in == 0) return;
+        if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+            if (detail::lockstep_lookups()) {
+                detail::launch_erase_lockstep<view_type, StatusHandler, detail::lockstep_keys<unsigned(cg_size())>()>(
+                    keys_in, num_in, view(), probing_length, status_out, stream);
+                return;
+            }
+        }
         kernels::erase<view_type, StatusHandler>
             <<<detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(keys_in, num_in, view(),
                                                                                 probing_length, status_out);

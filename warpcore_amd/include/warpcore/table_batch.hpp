@@ -17,8 +17,8 @@
 // fill), and one persistent kernel lets each cooperative group take items
 // from an atomic counter (work stealing): small and large tables mix
 // freely and no group idles while items remain. Inside an item the group
-// runs the lockstep kernels' per-range code against that table's view.
-// Launch parameters (keys per lockstep batch, item size, blocks per CU)
+// runs the refill kernels' per-range code against that table's view.
+// Launch parameters (keys in flight per group, item size, blocks per CU)
 // come from a BatchLaunch; the C ABI's tuner times candidates on the
 // caller's workload and keeps the fastest.
 #pragma once
@@ -28,6 +28,7 @@
 #include "defaults.hpp"
 #include "engine.hpp"
 #include "lockstep_probing.hpp"
+#include "refill_probing.hpp"
 #include "scan.hpp"
 #include "single_value_hash_table.hpp"
 #include "status.hpp"
@@ -52,7 +53,7 @@ WC_H index_t batch_table_windows(index_t min_capacity, index_t cg) noexcept {
 }
 
 struct BatchLaunch {
-    int keys_per_batch;   // U of the lockstep code: 4 or 8
+    int keys_per_batch;   // keys in flight per group (U of the refill code): 4 or 8
     index_t item_keys;    // keys per work item
     int blocks_per_cu;    // persistent grid: CUs * blocks_per_cu blocks of 256
 };
@@ -92,8 +93,12 @@ __global__ __launch_bounds__(256) void batch_persistent(
     index_t probing_length, typename StatusHandler::base_type* __restrict__ status_out) {
     constexpr unsigned CG = unsigned(View::cg_size());
     using W = std::uint32_t;
-    __shared__ detail::StagedKey<typename View::key_type, W> stage[256 / CG * U];
-    auto* my = stage + (threadIdx.x / CG) * U;
+    using Stage = typename std::conditional<IsInsert,
+                                            detail::RefillInsertEntry<typename View::key_type, W,
+                                                                      typename View::value_type>,
+                                            detail::RefillEntry<typename View::key_type, W>>::type;
+    __shared__ Stage stage[256];
+    auto* my = stage + (threadIdx.x & ~(CG - 1));
     const unsigned rank = lane_id() & (CG - 1);
     const index_t total = num_tables ? first_item[num_tables - 1] +
                                            SDIV(offsets[num_tables] - offsets[num_tables - 1], item_keys)
@@ -113,11 +118,11 @@ __global__ __launch_bounds__(256) void batch_persistent(
         const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, d.geo.windows);
         detail::DeferredJoin join;
         if constexpr (IsInsert)
-            detail::insert_lockstep_range<View, StatusHandler, U, W, true>(keys_in, values_in, begin, end, U, view,
-                                                                           budget, status_out, my, join);
+            detail::insert_refill_range<View, StatusHandler, U, W, true>(keys_in, values_in, begin, end, view, budget,
+                                                                         status_out, my, join);
         else
-            detail::retrieve_lockstep_range<View, StatusHandler, U, W>(keys_in, begin, end, U, values_out, view,
-                                                                       budget, status_out, my, join);
+            detail::retrieve_refill_range<View, StatusHandler, U, W>(keys_in, begin, end, values_out, view, budget,
+                                                                     status_out, my, join);
         const status_base_t all = tile<CG>::reduce_or(join.acc);
         if (rank == 0 && all != 0) statuses[t].atomic_join(Status::from_base(all));
     }

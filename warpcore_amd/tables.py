@@ -683,14 +683,15 @@ class Server:
     INSERT, RETRIEVE, ERASE = 0, 1, 2
 
     def __init__(self, table_capacities, key_bits: int = 64, value_bits: int = 64, max_requests: int = 0,
-                 item_keys: int = 0, max_load: float = 0.9, use_graph: bool = True, blocks_per_cu: int = 0):
+                 item_keys: int = 0, max_load: float = 0.9, use_graph: bool = True, blocks_per_cu: int = 0,
+                 max_step_keys: int = 0):
         import numpy as np
         self.key_bits, self.value_bits = key_bits, value_bits
         self._p = f"wc_server_{self._NAMES[(key_bits, value_bits)]}_"
         self._lib = _capi.lib()
         caps = np.ascontiguousarray(table_capacities, dtype=np.uint64)
         self._h = self._fn("create")(caps.ctypes.data_as(ctypes.c_void_p), len(caps), max_requests, item_keys,
-                                     max_load, int(use_graph), blocks_per_cu)
+                                     max_step_keys, max_load, int(use_graph), blocks_per_cu)
         if not self._h:
             raise ValueError("server could not be created")
 
