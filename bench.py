@@ -279,15 +279,18 @@ def run_serving(g: Gpu, num_tables: int = 64, total_bytes: float = 100e9, num_re
     del hot, take
     res = {"tables": num_tables, "table_slots": int(caps[0]), "requests": num_requests, "keys": n}
 
-    def one(mode, arrival):
+    def one(mode, arrival, count=num_requests):
         srv = Server(caps, 64, 64, **(server_kw or {}))
-        lat, mk = srv.drive(mode, ops, tabs, sizes, arrival, keys, values, out, st)
-        steps = srv.stats()["steps"] if mode == 0 else num_requests
+        lat, mk = srv.drive(mode, ops[:count], tabs[:count], sizes[:count], arrival[:count], keys, values, out, st)
+        steps = srv.stats()["steps"] if mode == 0 else count
         del srv
         torch.cuda.empty_cache()
         return lat, mk, steps
 
     zero = np.zeros(num_requests)
+    # warm-up of both paths (table allocation, clocks) before anything is timed
+    one(1, zero, 300)
+    one(0, zero, 300)
     lat_b, mk_b, _ = one(1, zero)
     lat_s, mk_s, steps_s = one(0, zero)
     res["throughput"] = {"server_gkeys_s": round(n / mk_s / 1e3, 3), "launch_per_request_gkeys_s":

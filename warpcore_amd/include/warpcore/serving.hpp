@@ -72,77 +72,102 @@ struct RequestDesc {
 
 namespace kernels {
 
+// Step preparation (one workgroup): copies the step's live descriptors from
+// the host-pinned staging buffer the scheduler wrote, counts each request's
+// work items and writes their exclusive scan (first item of each request).
 template<class Desc>
-__global__ __launch_bounds__(256) void serve_count_items(const Desc* __restrict__ reqs,
-                                                         const std::uint32_t* __restrict__ num_reqs,
-                                                         index_t max_reqs, index_t item_keys,
-                                                         index_t* __restrict__ items) {
-    const index_t r = index_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r < max_reqs) items[r] = r < *num_reqs ? SDIV(reqs[r].n, item_keys) : 0;
-}
-
-// Work-item listing: item i (of first[R-1] + items[R-1]) belongs to the last
-// request r with first[r] <= i (first is the exclusive scan of items).
-template<class T>
-__global__ __launch_bounds__(256) void serve_fill_items(const T* __restrict__ first, const T* __restrict__ items,
-                                                        index_t max_reqs, std::uint32_t* __restrict__ item_req) {
-    const T total = first[max_reqs - 1] + items[max_reqs - 1];
-    const index_t stride = index_t(gridDim.x) * blockDim.x;
-    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total; i += stride) {
-        index_t lo = 0, hi = max_reqs;   // upper_bound(first, i) - 1
-        while (hi - lo > 1) {
-            const index_t mid = (lo + hi) / 2;
-            if (first[mid] <= T(i)) lo = mid;
-            else hi = mid;
+__global__ __launch_bounds__(256) void serve_prepare(const Desc* __restrict__ staged,
+                                                     const std::uint32_t* __restrict__ staged_num, index_t max_reqs,
+                                                     index_t item_keys, Desc* __restrict__ reqs,
+                                                     index_t* __restrict__ first, index_t* __restrict__ state) {
+    const index_t num_raw = *reinterpret_cast<const volatile std::uint32_t*>(staged_num);
+    const index_t num = num_raw < max_reqs ? num_raw : max_reqs;
+    index_t carry = 0;
+    for (index_t base = 0; base < num; base += blockDim.x) {
+        const index_t r = base + threadIdx.x;
+        index_t k = 0;
+        if (r < num) {
+            const Desc d = staged[r];
+            reqs[r] = d;
+            k = SDIV(d.n, item_keys);
         }
-        item_req[i] = std::uint32_t(lo);
+        index_t total;
+        const index_t incl = detail::block_inclusive_scan(k, total);
+        if (r < num) first[r] = carry + incl - k;
+        carry += total;
+    }
+    if (threadIdx.x == 0) {
+        state[0] = num;     // live requests
+        state[1] = carry;   // work items
     }
 }
 
-// One step: group g of the grid takes items g, g + G, ... (items hold
-// item_keys keys each, so a static assignment balances). Lookups and
-// inserts use the refill kernels' range code, erases the lockstep one.
+// Wavefronts per SIMD the step kernel is compiled for (0: compiler's choice).
+#ifndef WARPCORE_SERVE_WAVES
+#define WARPCORE_SERVE_WAVES 0
+#endif
+#if WARPCORE_SERVE_WAVES > 0
+#define WC_SERVE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WARPCORE_SERVE_WAVES)))
+#else
+#define WC_SERVE_OCCUPANCY
+#endif
+
+// One step: group g of the (resident) grid takes work items g, g + G, ...
+// (items hold item_keys keys, so a static assignment balances and needs no
+// shared counter). An item's request is found by binary search in an LDS
+// copy of the requests' first items. Lookups and inserts use the refill
+// kernels' range code, erases the lockstep one.
 template<class View, int UR, int UI, int UE>
-__global__ __launch_bounds__(256) void serve_step(const serving::RequestDesc<typename View::key_type,
+__global__ __launch_bounds__(256) WC_SERVE_OCCUPANCY void serve_step(const serving::RequestDesc<typename View::key_type,
                                                                              typename View::value_type>* __restrict__ reqs,
-                                                  const std::uint32_t* __restrict__ item_req,
                                                   const index_t* __restrict__ first_item,
-                                                  const index_t* __restrict__ items, index_t max_reqs,
-                                                  index_t item_keys, const View* __restrict__ views,
-                                                  index_t probing_length) {
+                                                  index_t* __restrict__ state, index_t max_reqs, index_t item_keys,
+                                                  const View* __restrict__ views, index_t probing_length) {
     using K = typename View::key_type;
     using W = std::uint32_t;
     constexpr unsigned CG = unsigned(View::cg_size());
     __shared__ detail::StagedKey<K, W> stage[256 / CG * UE];
     __shared__ detail::RefillEntry<K, W> refill[256];
     __shared__ detail::RefillInsertEntry<K, W, typename View::value_type> refill_insert[256];
+    extern __shared__ index_t first_lds[];
     auto* my = stage + (threadIdx.x / CG) * UE;
     auto* my_refill = refill + (threadIdx.x & ~(CG - 1));
     auto* my_refill_insert = refill_insert + (threadIdx.x & ~(CG - 1));
     const unsigned rank = lane_id() & (CG - 1);
-    const index_t total = first_item[max_reqs - 1] + items[max_reqs - 1];
+    const index_t num = state[0] < max_reqs ? state[0] : max_reqs;
+    const index_t total = state[1];
+    for (index_t r = threadIdx.x; r < num; r += blockDim.x) first_lds[r] = first_item[r];
+    __syncthreads();
+    using H = status_handlers::ReturnStatus;
     const index_t gid = (index_t(blockIdx.x) * blockDim.x + threadIdx.x) / CG;
     const index_t groups = (index_t(gridDim.x) * blockDim.x) / CG;
-    using H = status_handlers::ReturnStatus;
     for (index_t item = gid; item < total; item += groups) {
-        const std::uint32_t r = item_req[item];
-        const auto d = reqs[r];
-        const index_t begin = (item - first_item[r]) * item_keys;
-        const index_t end = begin + item_keys < d.n ? begin + item_keys : d.n;
-        const View view = views[d.table];
-        const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, view.geo.windows);
-        detail::DeferredJoin join;
-        auto* st = reinterpret_cast<Status*>(d.status_out);
-        if (d.op == std::uint32_t(serving::Op::insert))
-            detail::insert_refill_range<View, H, UI, W, true>(d.keys, d.values, begin, end, view, budget, st,
-                                                              my_refill_insert, join);
-        else if (d.op == std::uint32_t(serving::Op::retrieve))
-            detail::retrieve_refill_range<View, H, UR, W>(d.keys, begin, end, d.values_out, view, budget, st,
-                                                          my_refill, join);
-        else
-            detail::erase_lockstep_range<View, H, UE, W>(d.keys, begin, end, UE, view, budget, st, my, join);
-        const status_base_t all = tile<CG>::reduce_or(join.acc);
-        if (rank == 0 && all != 0) view.status->atomic_join(Status::from_base(all));
+        {
+            index_t lo = 0, hi = num;   // last request whose first item is <= item
+            while (hi - lo > 1) {
+                const index_t mid = (lo + hi) / 2;
+                if (first_lds[mid] <= item) lo = mid;
+                else hi = mid;
+            }
+            const std::uint32_t r = std::uint32_t(lo);
+            const auto d = reqs[r];
+            const index_t begin = (item - first_lds[r]) * item_keys;
+            const index_t end = begin + item_keys < d.n ? begin + item_keys : d.n;
+            const View view = views[d.table];
+            const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, view.geo.windows);
+            detail::DeferredJoin join;
+            auto* st = reinterpret_cast<Status*>(d.status_out);
+            if (d.op == std::uint32_t(serving::Op::insert))
+                detail::insert_refill_range<View, H, UI, W, true>(d.keys, d.values, begin, end, view, budget, st,
+                                                                  my_refill_insert, join);
+            else if (d.op == std::uint32_t(serving::Op::retrieve))
+                detail::retrieve_refill_range<View, H, UR, W>(d.keys, begin, end, d.values_out, view, budget, st,
+                                                              my_refill, join);
+            else
+                detail::erase_lockstep_range<View, H, UE, W>(d.keys, begin, end, UE, view, budget, st, my, join);
+            const status_base_t all = tile<CG>::reduce_or(join.acc);
+            if (rank == 0 && all != 0) view.status->atomic_join(Status::from_base(all));
+        }
     }
 }
 
@@ -198,11 +223,9 @@ public:
             budget_.push_back(index_t(double(tables_.back()->capacity()) * cfg_.max_load));
             reserved_.push_back(0);
         }
-        max_items_ = SDIV(cfg_.max_step_keys, cfg_.item_keys) + cfg_.max_requests;
         std::vector<view_type> views;
         for (auto& t : tables_) views.push_back(t->view());
-        ok_ = alloc(&views_d_, views.size()) && alloc(&item_req_, max_items_) && alloc(&items_, cfg_.max_requests) &&
-              alloc(&first_, cfg_.max_requests) && alloc(&parts_, detail::kScanParts) && alloc(&num_reqs_d_, 1);
+        ok_ = alloc(&views_d_, views.size()) && alloc(&first_, cfg_.max_requests) && alloc(&state_, 2);
         for (int b = 0; b < 2 && ok_; ++b) {
             ok_ = alloc(&desc_d_[b], cfg_.max_requests) &&
                   hipHostMalloc(reinterpret_cast<void**>(&desc_h_[b]), sizeof(desc_type) * cfg_.max_requests) ==
@@ -216,19 +239,27 @@ public:
               hipStreamCreateWithFlags(&capture_, hipStreamNonBlocking) == hipSuccess &&
               hipMemcpy(views_d_, views.data(), sizeof(view_type) * views.size(), hipMemcpyHostToDevice) == hipSuccess;
         if (!ok_) return;
+        // persistent grid: as many blocks as can be resident (occupancy),
+        // at most blocks_per_cu per CU
+        int resident = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, step_kernel(), int(block_size()),
+                                                         sizeof(index_t) * cfg_.max_requests) != hipSuccess ||
+            resident < 1)
+            resident = 1;
+        step_grid_ = unsigned(detail::num_compute_units()) * unsigned(std::min(resident, cfg_.blocks_per_cu));
         worker_ = std::thread([this] { loop(); });
     }
     ~Server() {
         {
             std::lock_guard<std::mutex> g(mu_);
             stop_ = true;
+            stop_flag_.store(true);
         }
         cv_.notify_all();
         if (worker_.joinable()) worker_.join();
         if (stream_) (void)hipStreamSynchronize(stream_);
         for (auto& g : graphs_) if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        for (void* p : {(void*)views_d_, (void*)item_req_, (void*)items_, (void*)first_, (void*)parts_,
-                        (void*)num_reqs_d_, (void*)desc_d_[0], (void*)desc_d_[1]})
+        for (void* p : {(void*)views_d_, (void*)first_, (void*)state_, (void*)desc_d_[0], (void*)desc_d_[1]})
             if (p) (void)hipFree(p);
         for (int b = 0; b < 2; ++b) {
             if (desc_h_[b]) (void)hipHostFree(desc_h_[b]);
@@ -265,6 +296,7 @@ public:
                 finish_refused(*r);
             } else {
                 pending_.push_back(r);
+                waiting_.store(pending_.size(), std::memory_order_release);
             }
         }
         cv_.notify_all();
@@ -374,33 +406,33 @@ private:
             }
         }
         pending_.swap(keep);
+        waiting_.store(pending_.size(), std::memory_order_release);
     }
 
     struct Graph {
         hipGraphExec_t exec = nullptr;
     };
 
-    // Work-item listing plus the persistent kernel of one step, reading the
-    // descriptors at desc_d_[b] (fixed shape: max_requests descriptors, of
-    // which the first *num_reqs_d_ are live).
+    // One step: preparation, work-item listing and the persistent kernel,
+    // reading staging buffer b (fixed shape: max_requests descriptors, of
+    // which the first *num_reqs_h_[b] are live).
     void enqueue_step(int b, hipStream_t stream) {
         const index_t R = cfg_.max_requests;
-        const unsigned rg = unsigned(SDIV(R, index_t(256)));
-        const unsigned cus = unsigned(detail::num_compute_units());
-        kernels::serve_count_items<desc_type><<<rg, 256, 0, stream>>>(desc_d_[b], num_reqs_d_, R, cfg_.item_keys, items_);
-        detail::device_scan(items_, first_, R, parts_, false, stream);
-        kernels::serve_fill_items<index_t><<<cus * 4, 256, 0, stream>>>(first_, items_, R, item_req_);
-        const unsigned grid = unsigned(cus * cfg_.blocks_per_cu);
-        kernels::serve_step<view_type, detail::refill_keys<unsigned(CG)>(),
-                            detail::refill_insert_keys<unsigned(CG)>(), detail::lockstep_keys<unsigned(CG)>()>
-            <<<grid, block_size(), 0, stream>>>(desc_d_[b], item_req_, first_, items_, R, cfg_.item_keys, views_d_,
-                                                 defaults::probing_length());
+        kernels::serve_prepare<desc_type><<<1, 256, 0, stream>>>(desc_h_[b], num_reqs_h_[b], R, cfg_.item_keys,
+                                                                 desc_d_[b], first_, state_);
+        step_kernel()<<<step_grid_, block_size(), sizeof(index_t) * R, stream>>>(
+            desc_d_[b], first_, state_, R, cfg_.item_keys, views_d_, defaults::probing_length());
     }
 
-    void launch(int b, index_t count) {
-        // the request count and the live descriptors travel with the step
-        (void)hipMemcpyAsync(num_reqs_d_, num_reqs_h_[b], sizeof(std::uint32_t), hipMemcpyHostToDevice, stream_);
-        (void)hipMemcpyAsync(desc_d_[b], desc_h_[b], sizeof(desc_type) * count, hipMemcpyHostToDevice, stream_);
+    WC_H static auto step_kernel() noexcept {
+        return &kernels::serve_step<view_type, detail::refill_keys<unsigned(CG)>(),
+                                    detail::refill_insert_keys<unsigned(CG)>(), detail::lockstep_keys<unsigned(CG)>()>;
+    }
+
+    void launch(int b) {
+        // the step reads its request count and descriptors from the pinned
+        // staging buffer b, written by this thread before the launch
+        std::atomic_thread_fence(std::memory_order_release);
         if (!cfg_.use_graph) {
             enqueue_step(b, stream_);
             return;
@@ -469,25 +501,24 @@ private:
             if (!step.empty()) {
                 ++step_no;
                 *num_reqs_h_[b] = std::uint32_t(step.size());
-                launch(b, step.size());
+                launch(b);
                 hipEvent_t ev = nullptr;
                 (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
                 (void)hipEventRecord(ev, stream_);
                 inflight.push_back(InFlight{ev, step_no, std::move(step)});
                 b ^= 1;
             }
-            // retire finished steps; with a free descriptor buffer, go back
-            // and build the next step as soon as requests are pending
+            // retire finished steps; with a free staging buffer, go back and
+            // build the next step as soon as requests are pending (spinning:
+            // the scheduler thread owns its core)
             while (!inflight.empty()) {
                 if (hipEventQuery(inflight.front().ev) == hipSuccess) {
                     retire(inflight);
                     continue;
                 }
-                if (inflight.size() < 2) {
-                    std::unique_lock<std::mutex> g(mu_);
-                    if (!pending_.empty() || stop_) break;
-                    cv_.wait_for(g, std::chrono::microseconds(20), [&] { return !pending_.empty() || stop_; });
-                    if (!pending_.empty() || stop_) break;
+                if (inflight.size() < 2) {   // poll: a new request or the step's end, whichever comes first
+                    if (waiting_.load(std::memory_order_acquire) != 0 || stop_flag_.load()) break;
+                    std::this_thread::yield();
                     continue;
                 }
                 (void)hipEventSynchronize(inflight.front().ev);
@@ -498,16 +529,14 @@ private:
 
     Config cfg_;
     bool ok_ = true;
-    index_t max_items_ = 0;
     std::vector<std::unique_ptr<table_type>> tables_;
     std::vector<index_t> budget_, reserved_;
     view_type* views_d_ = nullptr;
     desc_type* desc_d_[2] = {nullptr, nullptr};
     desc_type* desc_h_[2] = {nullptr, nullptr};
     std::uint32_t* num_reqs_h_[2] = {nullptr, nullptr};
-    std::uint32_t* num_reqs_d_ = nullptr;
-    std::uint32_t* item_req_ = nullptr;
-    index_t *items_ = nullptr, *first_ = nullptr, *parts_ = nullptr;
+    index_t *first_ = nullptr, *state_ = nullptr;   // state: live requests, work items
+    unsigned step_grid_ = 0;
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;
     hipStream_t capture_ = nullptr;
@@ -518,6 +547,8 @@ private:
     std::vector<std::shared_ptr<Request>> all_;
     index_t next_id_ = 0;
     bool stop_ = false;
+    std::atomic<bool> stop_flag_{false};   // stop_, readable without mu_
+    std::atomic<std::size_t> waiting_{0};  // pending_.size(), readable without mu_
     Stats stats_;
     std::thread worker_;
 };
