@@ -557,6 +557,21 @@ WC_API int wc_gen_zipf_u64(uint64_t* out, uint64_t n, uint64_t num_distinct, dou
 WC_API int wc_gen_interleave_u32(uint32_t* out, const uint32_t* a, const uint32_t* b, uint64_t half, uint64_t seed, void* stream);
 WC_API int wc_gen_interleave_u64(uint64_t* out, const uint64_t* a, const uint64_t* b, uint64_t half, uint64_t seed, void* stream);
 
+/* Key files: raw little-endian arrays of 4- or 8-byte keys, the input format
+ * of the reference benchmarks' load_keys / helpers::load_binary
+ * (benchmarks/include/benchmark_common.cuh:62-75). At most max_keys keys
+ * are read from the start of the file (fewer if it is shorter).
+ *   _load_binary_*  into a HOST buffer of max_keys keys; returns the count
+ *                   read, or -1 when the file cannot be read
+ *   _load_keys_*    into a DEVICE buffer of max_keys keys; *num_out = count
+ *                   read; returns 0, -2 (unreadable file / bad argument) or
+ *                   a hipError_t. Unlike the reference, only the keys read
+ *                   are copied. */
+WC_API int64_t wc_load_binary_u32(const char* path, uint64_t max_keys, uint32_t* keys_out);
+WC_API int64_t wc_load_binary_u64(const char* path, uint64_t max_keys, uint64_t* keys_out);
+WC_API int wc_load_keys_u32(const char* path, uint64_t max_keys, uint32_t* keys_d, uint64_t* num_out, void* stream);
+WC_API int wc_load_keys_u64(const char* path, uint64_t max_keys, uint64_t* keys_d, uint64_t* num_out, void* stream);
+
 /* Device copy of `bytes` (a multiple of 16, 16-byte aligned pointers), 16
  * bytes per lane: a streaming kernel of known traffic, used to calibrate
  * memory-traffic counters. No reference counterpart. */

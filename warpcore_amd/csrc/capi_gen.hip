@@ -1,7 +1,10 @@
 // extern "C" generators of synthetic inputs (declared in include/warpcore_c.h).
 #include "capi_common.hpp"
 
+#include <warpcore/io.hpp>
 #include <warpcore/random_distributions.hpp>
+
+#include <algorithm>
 
 using namespace warpcore;
 using wc_capi::as_stream;
@@ -107,3 +110,22 @@ extern "C" int wc_get_group_kernel(void) { return detail::group_kernel(); }
 // (1, default) or insert every occurrence on its own (0).
 extern "C" void wc_set_counting_combine(int mode) { detail::set_counting_combine(mode); }
 extern "C" int wc_get_counting_combine(void) { return detail::counting_combine(); }
+
+// Key files (warpcore/io.hpp): raw little-endian key arrays.
+#define WC_KEYFILE_API(SFX, K)                                                                                    \
+    extern "C" int64_t wc_load_binary_##SFX(const char* path, uint64_t max_keys, K* keys_out) {                  \
+        std::vector<K> keys;                                                                                      \
+        if (!keys_out && max_keys) return -1;                                                                     \
+        if (!io::load_binary<K>(path, max_keys, keys)) return -1;                                                 \
+        std::copy(keys.begin(), keys.end(), keys_out);                                                            \
+        return int64_t(keys.size());                                                                              \
+    }                                                                                                             \
+    extern "C" int wc_load_keys_##SFX(const char* path, uint64_t max_keys, K* keys_d, uint64_t* num_out,         \
+                                     void* stream) {                                                              \
+        if (!keys_d && max_keys) return wc_capi::kBadArgument;                                                    \
+        const hipError_t e = io::load_keys<K>(path, max_keys, keys_d, num_out, as_stream(stream));                \
+        return e == hipErrorInvalidValue ? wc_capi::kBadArgument : int(e);                                        \
+    }
+
+WC_KEYFILE_API(u32, uint32_t)
+WC_KEYFILE_API(u64, uint64_t)

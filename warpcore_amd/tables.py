@@ -752,3 +752,28 @@ class Server:
                                  _ptr(values), _ptr(values_out), _ptr(status_out), vp(lat), ctypes.byref(mk)),
                "drive")
         return lat, mk.value
+
+
+def load_binary(path, key_bits: int = 64, max_keys: int = 1 << 27):
+    """Keys of a raw little-endian key file (the reference benchmarks'
+    load_keys input format, benchmarks/include/benchmark_common.cuh:62-75),
+    at most max_keys from the start, as a numpy array (host)."""
+    import numpy as np
+    dt = {32: np.uint32, 64: np.uint64}[key_bits]
+    out = np.empty(max_keys, dt)
+    fn = getattr(_capi.lib(), f"wc_load_binary_u{key_bits}")
+    n = fn(str(path).encode(), max_keys, out.ctypes.data_as(ctypes.c_void_p))
+    if n < 0:
+        raise OSError(f"cannot read key file {path}")
+    return out[:n]
+
+
+def load_keys(path, key_bits: int = 64, max_keys: int = 1 << 27, stream=None):
+    """The same keys copied to a device tensor of the keys read (int32 or
+    int64 holding the key bits)."""
+    import torch
+    buf = torch.empty(max_keys, dtype={32: torch.int32, 64: torch.int64}[key_bits], device="cuda")
+    num = ctypes.c_uint64(0)
+    fn = getattr(_capi.lib(), f"wc_load_keys_u{key_bits}")
+    _check(fn(str(path).encode(), max_keys, _ptr(buf), ctypes.byref(num), _stream(stream)), "load_keys")
+    return buf[: num.value]
