@@ -309,6 +309,31 @@ def run_serving(g: Gpu, num_tables: int = 64, total_bytes: float = 100e9, num_re
     return res
 
 
+def run_line_roofline(g: Gpu, table_bytes: int, lines: int = 1 << 28, reps: int = 3):
+    """Rate at which random 128-byte lines of a table_bytes footprint are
+    served (wc_util_gather_lines: 8 lanes per line, 8 lines in flight per
+    group), the practical ceiling of the probing kernels at that size."""
+    torch = g.torch
+    buf = torch.empty(table_bytes // 16 * 2, dtype=torch.int64, device="cuda")
+    buf.fill_(7)
+    sink = torch.zeros(1, dtype=torch.int32, device="cuda")
+    g.stream.wait_stream(torch.cuda.current_stream())
+    best = float("inf")
+    for r in range(reps + 1):
+        e0, e1 = g.event(), g.event()
+        e0.record(g.stream)
+        g.check(g.lib.wc_util_gather_lines(g.p(buf), table_bytes, lines, 0x51 + r, g.p(sink), g.sp), "gather")
+        e1.record(g.stream)
+        g.stream.synchronize()
+        if r:
+            best = min(best, e0.elapsed_time(e1))
+    del buf, sink
+    torch.cuda.empty_cache()
+    glines = lines / (best * 1e-3) / 1e9
+    return {"glines_s": round(glines, 2), "gbs": round(glines * LINE, 1), "footprint_bytes": table_bytes,
+            "ms": round(best, 3)}
+
+
 def cpu_baseline(log2n: int = 24, reps: int = 3):
     """OpenMP table with the same hashing, probing and CAS protocol, on a
     2^log2n-key sample of the same workload (u64/u64, LF 0.9, 50 % hits)."""
@@ -413,6 +438,14 @@ def main():
                 "other_kernel": {"kernel": f"svht_u64u64_{other}", "achieved": round(achieved_other, 1),
                                  "frac": round(achieved_other / HBM_PEAK_GBS, 4), "bytes_per_op": per_op[other],
                                  "avg_kernel_ms": round(kt[other], 4), "traffic": read_traffic(other)}}
+    # practical ceiling for random whole lines at the headline table's size
+    lr = run_line_roofline(g, r64["table_bytes"])
+    tr = read_traffic("retrieve")
+    if tr:
+        lines_per_lookup = tr / LINE / n
+        lr["retrieve_lines_per_op"] = round(lines_per_lookup, 3)
+        lr["retrieve_frac"] = round(n / (kt["retrieve"] * 1e-3) * lines_per_lookup / 1e9 / lr["glines_s"], 4)
+    log(f"[bench] line roofline: {lr}")
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
@@ -441,6 +474,7 @@ def main():
                    "retrieve_gops": round(n / r64["retrieve_ms"] / 1e6, 3),
                    "kernel_only_gops": round(2 * n / (r64["insert_ms"] + r64["retrieve_ms"]) / 1e6, 3)},
         "roofline": roofline,
+        "line_roofline": lr,
         "cpu_baseline": cpu,
     }
     out.update(extras)

@@ -576,6 +576,12 @@ WC_API int wc_load_keys_u64(const char* path, uint64_t max_keys, uint64_t* keys_
  * bytes per lane: a streaming kernel of known traffic, used to calibrate
  * memory-traffic counters. No reference counterpart. */
 WC_API int wc_util_copy(void* dst, const void* src, uint64_t bytes, void* stream);
+/* Reads `lines` random 128-byte lines of the first table_bytes bytes at
+ * `table` (16-byte aligned), 8 lanes per line and 8 lines in flight per
+ * group: the rate the memory system serves random whole lines at this
+ * footprint, the practical ceiling of the probing kernels. `sink` is a
+ * device word the kernel may write. No reference counterpart. */
+WC_API int wc_util_gather_lines(const void* table, uint64_t table_bytes, uint64_t lines, uint64_t seed, uint32_t* sink, void* stream);
 
 /* Capacity rule detail::get_valid_capacity (include/warpcore/base.cuh:27-34)
  * and the unique-key map, evaluated on the host. */
