@@ -112,13 +112,18 @@ WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys
     index_t next = 0;          // keys of the range staged so far (group-uniform)
     std::uint64_t avail = 0;   // stage entries holding a key not yet taken (group-uniform)
     std::uint32_t stage_pos = 0;
+    // the keys of the next stage are loaded one stage ahead, so staging
+    // never waits for memory
+    K ahead = K{};
+    if (len > 0) ahead = keys_in[first + (rank < len ? rank : 0)];
 
     while (true) {
         // stage the next CG keys of the range once the stage is used up
         if (avail == 0 && next < len) {
             const bool in = next + rank < len;
             const index_t i = first + next + (in ? rank : 0);
-            const K k = keys_in[i];
+            const K k = ahead;
+            if (next + CG < len) ahead = keys_in[first + next + CG + (next + CG + rank < len ? rank : 0)];
             const bool valid = in && core.is_valid_key(k);
             if (valid) {
                 Cursor c;
@@ -254,8 +259,13 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
     V val[U];
     W w[U], step[U];
     std::uint32_t pos[U], rnd[U];
-    std::uint64_t emp[U];   // group bits: EMPTY lanes of the current window not tried yet
-    Probe seen[U];          // this lane's slot as read in the current window
+    // When a slot reads as just its key (no packed value bits), a claiming
+    // lane always saw EMPTY there, so the CAS's expected word is the EMPTY
+    // probe and the window read need not be kept.
+    constexpr bool keep_seen = !std::is_same<Probe, K>::value;
+    using Bits = typename std::conditional<(CG <= 32), std::uint32_t, std::uint64_t>::type;
+    Bits emp[U];            // group bits: EMPTY lanes of the current window not tried yet
+    Probe seen[keep_seen ? U : 1];   // this lane's slot as read in the current window
     std::uint64_t live[U], claim[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -266,7 +276,7 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
         pos[u] = 0;
         rnd[u] = 0;
         emp[u] = 0;
-        seen[u] = Probe{};
+        if constexpr (keep_seen) seen[u] = Probe{};
         live[u] = 0;
         claim[u] = 0;
     }
@@ -274,12 +284,26 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
     index_t next = 0;
     std::uint64_t avail = 0;
     std::uint32_t stage_pos = 0;
+    // keys and values of the next stage, loaded one stage ahead
+    K ahead = K{};
+    V ahead_value = V{};
+    if (len > 0) {
+        const index_t i0 = first + (rank < len ? rank : 0);
+        ahead = keys_in[i0];
+        if constexpr (HasValues) ahead_value = values_in[i0];
+    }
 
     while (true) {
         if (avail == 0 && next < len) {
             const bool in = next + rank < len;
             const index_t i = first + next + (in ? rank : 0);
-            const K k = keys_in[i];
+            const K k = ahead;
+            const V kv = ahead_value;
+            if (next + CG < len) {
+                const index_t i2 = first + next + CG + (next + CG + rank < len ? rank : 0);
+                ahead = keys_in[i2];
+                if constexpr (HasValues) ahead_value = values_in[i2];
+            }
             const bool valid = in && core.is_valid_key(k);
             if (valid) {
                 Cursor c;
@@ -288,8 +312,7 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
                 e.key = k;
                 e.w = W(c.w);
                 e.step = W(c.step);
-                if constexpr (HasValues) e.value = values_in[i];
-                else e.value = V{};
+                e.value = kv;
                 stage[rank] = e;
             }
             if (in && !valid) {
@@ -339,7 +362,11 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
         for (int u = 0; u < U; ++u) {
             lead[u] = mine(claim[u]) && rank == unsigned(ffs64(emp[u]) - 1);
             old[u] = typename Core::cas_word{};
-            if (lead[u]) old[u] = core.cas_issue(slots + index_t(w[u]) * CG + rank, seen[u], key[u], val[u]);
+            if (lead[u]) {
+                Probe expect = Core::empty_probe();
+                if constexpr (keep_seen) expect = seen[u];
+                old[u] = core.cas_issue(slots + index_t(w[u]) * CG + rank, expect, key[u], val[u]);
+            }
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -350,16 +377,25 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
             const std::uint64_t hitl = ballot(tk == key[u]) & loading;
             const std::uint64_t ghit = spread(hitl);
             const std::uint64_t gemp = spread(ballot(Core::is_empty_key(tk)) & loading) & ~ghit;
-            if (mine(loading)) seen[u] = fresh[u];
-            if (mine(gemp)) emp[u] = group_bits(ballot(Core::is_empty_key(tk)));
+            if constexpr (keep_seen)
+                if (mine(loading)) seen[u] = fresh[u];
+            if (mine(gemp)) emp[u] = Bits(group_bits(ballot(Core::is_empty_key(tk))));
             // claiming slots: the leader's CAS outcome
-            const int outcome = lead[u] ? core.cas_outcome(old[u], seen[u], key[u]) : -1;
+            int outcome = -1;
+            if (lead[u]) {
+                if constexpr (keep_seen) {
+                    outcome = core.cas_outcome(old[u], seen[u], key[u]);
+                } else {
+                    Probe expect = Core::empty_probe();
+                    outcome = core.cas_outcome(old[u], expect, key[u]);
+                }
+            }
             const std::uint64_t won = spread(ballot(outcome == 1)) & claiming;
             const std::uint64_t dupc = spread(ballot(outcome == 2)) & claiming;
             const std::uint64_t retry = spread(ballot(outcome == 3)) & claiming;
             const std::uint64_t lost = claiming & ~won & ~dupc & ~retry;
             const unsigned leader = unsigned(ffs64(emp[u]) - 1);
-            if (mine(lost)) emp[u] &= emp[u] - 1;
+            if (mine(lost)) emp[u] &= Bits(emp[u] - 1);
             const std::uint64_t drained = lost & ballot(emp[u] == 0);
             // finished keys: duplicate (found in the window or returned by
             // the CAS) or claimed
@@ -453,30 +489,46 @@ __global__ __launch_bounds__(256) void insert_refill(const typename Core::key_ty
 
 namespace detail {
 
+// Grid of a refill launch: every block resident at once (the occupancy the
+// device reports for the kernel, at most 8 blocks of 256 per CU), since
+// each group owns an equal contiguous share of the keys and a second round
+// of blocks would run at lower occupancy. Queried once per kernel.
+template<auto Kernel>
+WC_H unsigned refill_grid() noexcept {
+    static const unsigned grid = [] {
+        int blocks = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, Kernel, int(block_size()), 0) != hipSuccess ||
+            blocks < 1)
+            blocks = 1;
+        return unsigned(num_compute_units()) * unsigned(blocks < 8 ? blocks : 8);
+    }();
+    return grid;
+}
+
 template<class Core, class StatusHandler, int U, bool HasValues>
 WC_H void launch_insert_refill(const typename Core::key_type* keys_in, const typename Core::value_type* values_in,
                                index_t num_in, const Core& core, index_t probing_length,
                                typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
-    const unsigned grid = unsigned(num_compute_units() * 8);
-    if (core.geometry().windows < (index_t(1) << 31))
-        kernels::insert_refill<Core, StatusHandler, U, std::uint32_t, HasValues>
-            <<<grid, block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out);
-    else
-        kernels::insert_refill<Core, StatusHandler, U, index_t, HasValues>
-            <<<grid, block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out);
+    if (core.geometry().windows < (index_t(1) << 31)) {
+        constexpr auto k = kernels::insert_refill<Core, StatusHandler, U, std::uint32_t, HasValues>;
+        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out);
+    } else {
+        constexpr auto k = kernels::insert_refill<Core, StatusHandler, U, index_t, HasValues>;
+        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out);
+    }
 }
 
 template<class Core, class StatusHandler, int U>
 WC_H void launch_retrieve_refill(const typename Core::key_type* keys_in, index_t num_in,
                                  typename Core::out_type* values_out, const Core& core, index_t probing_length,
                                  typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
-    const unsigned grid = unsigned(num_compute_units() * 8);
-    if (core.geometry().windows < (index_t(1) << 31))
-        kernels::retrieve_refill<Core, StatusHandler, U, std::uint32_t>
-            <<<grid, block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
-    else
-        kernels::retrieve_refill<Core, StatusHandler, U, index_t>
-            <<<grid, block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
+    if (core.geometry().windows < (index_t(1) << 31)) {
+        constexpr auto k = kernels::retrieve_refill<Core, StatusHandler, U, std::uint32_t>;
+        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
+    } else {
+        constexpr auto k = kernels::retrieve_refill<Core, StatusHandler, U, index_t>;
+        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
+    }
 }
 
 }  // namespace detail
