@@ -376,11 +376,13 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
             const K tk = Core::probe_key(fresh[u]);
             const std::uint64_t hitl = ballot(tk == key[u]) & loading;
             const std::uint64_t ghit = spread(hitl);
-            const std::uint64_t gemp = spread(ballot(Core::is_empty_key(tk)) & loading) & ~ghit;
+            const std::uint64_t em = ballot(Core::is_empty_key(tk));
+            const std::uint64_t gemp = spread(em & loading) & ~ghit;
             if constexpr (keep_seen)
                 if (mine(loading)) seen[u] = fresh[u];
-            if (mine(gemp)) emp[u] = Bits(group_bits(ballot(Core::is_empty_key(tk))));
-            // claiming slots: the leader's CAS outcome
+            // claiming slots: the leader's CAS outcome, which every lane of
+            // the group reads from the leader (one LDS-crossbar permute)
+            const unsigned leader = unsigned(ffs64(emp[u]) - 1);
             int outcome = -1;
             if (lead[u]) {
                 if constexpr (keep_seen) {
@@ -390,11 +392,13 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
                     outcome = core.cas_outcome(old[u], expect, key[u]);
                 }
             }
-            const std::uint64_t won = spread(ballot(outcome == 1)) & claiming;
-            const std::uint64_t dupc = spread(ballot(outcome == 2)) & claiming;
-            const std::uint64_t retry = spread(ballot(outcome == 3)) & claiming;
+            const int code = __builtin_amdgcn_ds_bpermute(int((shift + leader) << 2), outcome);
+            const std::uint64_t won = ballot(code == 1) & claiming;
+            const std::uint64_t dupc = ballot(code == 2) & claiming;
+            std::uint64_t retry = 0;   // only slots whose window read carries value bits
+            if constexpr (keep_seen) retry = ballot(code == 3) & claiming;
+            if (mine(gemp)) emp[u] = Bits(group_bits(em));
             const std::uint64_t lost = claiming & ~won & ~dupc & ~retry;
-            const unsigned leader = unsigned(ffs64(emp[u]) - 1);
             if (mine(lost)) emp[u] &= Bits(emp[u] - 1);
             const std::uint64_t drained = lost & ballot(emp[u] == 0);
             // finished keys: duplicate (found in the window or returned by
