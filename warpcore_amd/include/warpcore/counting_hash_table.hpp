@@ -55,10 +55,20 @@ struct CountingView {
         return old == seen ? 1 : (old == key_in ? 2 : 0);
     }
     // counting_hash_table.cuh:154-170
+    // With 64-bit counts and the default max_count (the largest count) the
+    // check before the add and the wrap guard after it can only act after
+    // 2^64 increments of one key, so the add is issued without reading the
+    // count and without waiting for the old value.
     template<class Join>
     WC_D void finish_insert(index_type slot, const value_type& inc, bool, Status& status, Join& join) const noexcept {
         const value_type add = AddValue ? inc : value_type(1);
         value_type* v = &(base.slots[slot].value);
+        if constexpr (sizeof(value_type) == 8) {
+            if (max_count == std::numeric_limits<value_type>::max()) {
+                __hip_atomic_fetch_add(v, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                return;
+            }
+        }
         if (atomic_load_relaxed(v) < max_count) {
             const value_type old = atomic_add(v, add);
             if (old > std::numeric_limits<value_type>::max() - add) {   // the counter wrapped
