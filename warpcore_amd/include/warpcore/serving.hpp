@@ -6,7 +6,7 @@
 // values, outputs); a scheduler thread admits them, coalesces the admitted
 // requests into steps, and runs a step as one fixed-shape sequence of
 // launches (work-item listing and one persistent kernel over small work
-// items of every request in the step), captured once as a HIP graph and
+// items of every request in the step), built once as a HIP graph and
 // replayed; only the request descriptors are uploaded per step. Steps are
 // issued back to back on one stream while the host already builds the next
 // one, so requests that arrive while a step runs join the following step; a
@@ -195,7 +195,7 @@ public:
         index_t max_step_keys = index_t(1) << 24;    // keys per step
         float max_load = 0.9f;                       // admission: slot budget per table
         int blocks_per_cu = 8;                       // persistent grid
-        bool use_graph = true;                       // replay a captured step
+        bool use_graph = true;                       // replay the step as a HIP graph
     };
 
     struct Request {
@@ -236,7 +236,6 @@ public:
         // blocking streams: work the client queued on the default stream
         // (filling its buffers) is ordered before the steps
         ok_ = hipStreamCreate(&stream_) == hipSuccess && hipStreamCreate(&side_) == hipSuccess &&
-              hipStreamCreateWithFlags(&capture_, hipStreamNonBlocking) == hipSuccess &&
               hipMemcpy(views_d_, views.data(), sizeof(view_type) * views.size(), hipMemcpyHostToDevice) == hipSuccess;
         if (!ok_) return;
         // persistent grid: as many blocks as can be resident (occupancy),
@@ -267,7 +266,6 @@ public:
         }
         if (stream_) (void)hipStreamDestroy(stream_);
         if (side_) (void)hipStreamDestroy(side_);
-        if (capture_) (void)hipStreamDestroy(capture_);
     }
 
     bool ok() const noexcept { return ok_; }
@@ -429,6 +427,46 @@ private:
                                     detail::refill_insert_keys<unsigned(CG)>(), detail::lockstep_keys<unsigned(CG)>()>;
     }
 
+    // The step as a HIP graph built node by node (no stream capture, so no
+    // other thread's device calls can meet a capture in progress): the
+    // prepare kernel, then the step kernel after it.
+    bool build_graph(int b) {
+        hipGraph_t g = nullptr;
+        if (hipGraphCreate(&g, 0) != hipSuccess) return false;
+        const desc_type* staged = desc_h_[b];
+        const std::uint32_t* staged_num = num_reqs_h_[b];
+        index_t R = cfg_.max_requests, item_keys = cfg_.item_keys, probing_length = defaults::probing_length();
+        desc_type* reqs = desc_d_[b];
+        const desc_type* reqs_c = desc_d_[b];
+        index_t* first = first_;
+        const index_t* first_c = first_;
+        index_t* state = state_;
+        const view_type* views = views_d_;
+        void* prep_args[] = {&staged, &staged_num, &R, &item_keys, &reqs, &first, &state};
+        void* step_args[] = {&reqs_c, &first_c, &state, &R, &item_keys, &views, &probing_length};
+        hipKernelNodeParams prep{};
+        prep.func = reinterpret_cast<void*>(kernels::serve_prepare<desc_type>);
+        prep.gridDim = dim3(1);
+        prep.blockDim = dim3(256);
+        prep.sharedMemBytes = 0;
+        prep.kernelParams = prep_args;
+        prep.extra = nullptr;
+        hipKernelNodeParams step{};
+        step.func = reinterpret_cast<void*>(step_kernel());
+        step.gridDim = dim3(step_grid_);
+        step.blockDim = dim3(unsigned(block_size()));
+        step.sharedMemBytes = unsigned(sizeof(index_t) * cfg_.max_requests);
+        step.kernelParams = step_args;
+        step.extra = nullptr;
+        hipGraphNode_t n_prep = nullptr, n_step = nullptr;
+        bool ok = hipGraphAddKernelNode(&n_prep, g, nullptr, 0, &prep) == hipSuccess &&
+                  hipGraphAddKernelNode(&n_step, g, &n_prep, 1, &step) == hipSuccess &&
+                  hipGraphInstantiate(&graphs_[b].exec, g, nullptr, nullptr, 0) == hipSuccess;
+        (void)hipGraphDestroy(g);
+        if (!ok) graphs_[b].exec = nullptr;
+        return ok;
+    }
+
     void launch(int b) {
         // the step reads its request count and descriptors from the pinned
         // staging buffer b, written by this thread before the launch
@@ -437,24 +475,10 @@ private:
             enqueue_step(b, stream_);
             return;
         }
-        if (!graphs_[b].exec) {
-            // captured on a non-blocking stream that never runs work, so
-            // that clients using the legacy default stream meanwhile do not
-            // collide with a capture in progress
-            hipGraph_t g = nullptr;
-            bool ok = hipStreamBeginCapture(capture_, hipStreamCaptureModeThreadLocal) == hipSuccess;
-            if (ok) {
-                enqueue_step(b, capture_);
-                ok = hipStreamEndCapture(capture_, &g) == hipSuccess &&
-                     hipGraphInstantiate(&graphs_[b].exec, g, nullptr, nullptr, 0) == hipSuccess;
-            }
-            if (g) (void)hipGraphDestroy(g);
-            if (!ok) {
-                graphs_[b].exec = nullptr;
-                cfg_.use_graph = false;
-                enqueue_step(b, stream_);
-                return;
-            }
+        if (!graphs_[b].exec && !build_graph(b)) {
+            cfg_.use_graph = false;
+            enqueue_step(b, stream_);
+            return;
         }
         (void)hipGraphLaunch(graphs_[b].exec, stream_);
     }
@@ -539,7 +563,6 @@ private:
     unsigned step_grid_ = 0;
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;
-    hipStream_t capture_ = nullptr;
     Graph graphs_[2];
     std::mutex mu_;
     std::condition_variable cv_, done_cv_;
