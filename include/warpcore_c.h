@@ -589,17 +589,21 @@ WC_API uint64_t wc_get_valid_capacity(uint64_t min_capacity, uint64_t cg_size);
 WC_API uint64_t wc_unique_key_u64(uint64_t offset, uint64_t i);
 WC_API uint32_t wc_unique_key_u32(uint64_t offset, uint64_t i);
 
-/* Kernel-shape policy (no reference counterpart): bulk operations on tables
- * whose slots take at most `bytes` are probed one lane per key, larger
- * tables by cooperative groups of cg_size lanes per key. */
+/* Kernel-shape policy (no reference counterpart): bulk lookups on tables
+ * whose slots take at most `bytes` (default 128 MiB) are probed one lane per
+ * key, larger tables by cooperative groups of cg_size lanes per key; bulk
+ * inserts likewise with their own limit (default 0: always groups). */
 WC_API void wc_set_lane_table_bytes(uint64_t bytes);
 WC_API uint64_t wc_get_lane_table_bytes(void);
+WC_API void wc_set_lane_insert_table_bytes(uint64_t bytes);
+WC_API uint64_t wc_get_lane_insert_table_bytes(void);
 
 /* Bulk kernels of tables probed by cooperative groups (no reference
- * counterpart), a bit mask: bit 0 = lockstep inserts, bit 1 = lockstep
- * lookups; a clear bit selects the streaming per-key state machine. Default
- * 1 (lockstep inserts, streaming lookups). All choices give identical
- * results. */
+ * counterpart), a bit mask: bit 3 = refill inserts, bit 2 = refill lookups,
+ * bit 0 = lockstep inserts, bit 1 = lockstep lookups and erases (a refill
+ * bit wins over the lockstep one; no bit = the streaming per-key state
+ * machine). Default 12 (refill inserts and lookups). All choices give
+ * identical results. */
 WC_API void wc_set_group_kernel(int kernel);
 WC_API int wc_get_group_kernel(void);
 

@@ -91,12 +91,15 @@ class kernel_shape:
     def __enter__(self):
         from warpcore_amd import _capi
         self.lib = _capi.lib()
-        self.saved = self.lib.wc_get_lane_table_bytes(), self.lib.wc_get_group_kernel()
+        self.saved = (self.lib.wc_get_lane_table_bytes(), self.lib.wc_get_group_kernel(),
+                      self.lib.wc_get_lane_insert_table_bytes())
         self.lib.wc_set_lane_table_bytes(self.limit)
+        self.lib.wc_set_lane_insert_table_bytes(self.limit)
         self.lib.wc_set_group_kernel(self.group_kernel)
         return self
 
     def __exit__(self, *exc):
         self.lib.wc_set_lane_table_bytes(self.saved[0])
+        self.lib.wc_set_lane_insert_table_bytes(self.saved[2])
         self.lib.wc_set_group_kernel(self.saved[1])
         return False

@@ -137,6 +137,8 @@ extern "C" uint32_t wc_unique_key_u32(uint64_t offset, uint64_t i) { return rand
 // one lane per key, larger ones by cooperative groups.
 extern "C" void wc_set_lane_table_bytes(uint64_t bytes) { detail::set_lane_table_bytes(bytes); }
 extern "C" uint64_t wc_get_lane_table_bytes(void) { return detail::lane_table_bytes(); }
+extern "C" void wc_set_lane_insert_table_bytes(uint64_t bytes) { detail::set_lane_insert_table_bytes(bytes); }
+extern "C" uint64_t wc_get_lane_insert_table_bytes(void) { return detail::lane_insert_table_bytes(); }
 
 // Kernel used for bulk lookups probed by cooperative groups: 0 = streaming
 // state machine, 1 = lockstep batches.

@@ -86,22 +86,39 @@ WC_H unsigned grid_for_batched(index_t items, index_t cg) noexcept {
 #endif
 constexpr int keys_per_lane() noexcept { return WARPCORE_KEYS_PER_LANE; }
 
-// Largest table (bytes of slots) probed lane-per-key. One lane reading a
-// whole 128-byte window issues 8 loads, each translating 64 addresses, so
-// beyond a few GB the address translation caches miss and 8-lane
-// cooperative windows (one translation per line) are faster.
-// The limit is a process-wide setting (set_lane_table_bytes) so a tuner or
-// a test can move it.
+// Largest table (bytes of slots) whose lookups are probed lane-per-key, and
+// the same for inserts. One lane reading a whole 128-byte window issues 8
+// loads, each translating 64 addresses, so on larger tables the address
+// translation caches miss and cooperative windows (one translation per
+// line) are faster. Measured on MI355X at load factor 0.9: lane lookups win
+// up to about 75 MB of 16-byte slots (2^22 keys: 25.2 vs 24.7 Gops/s; 2^20:
+// 29.4 vs 25.9) and lose from 300 MB on (2^24: 14.9 vs 19.8); lane inserts
+// lose at every size (2^20: 8.6 vs 11.9 Gops/s). Both limits are process-
+// wide settings (set_lane_table_bytes, set_lane_insert_table_bytes) so a
+// tuner or a test can move them.
 #ifndef WARPCORE_LANE_TABLE_BYTES
-#define WARPCORE_LANE_TABLE_BYTES (std::uint64_t(1) << 31)
+#define WARPCORE_LANE_TABLE_BYTES (std::uint64_t(1) << 27)
+#endif
+#ifndef WARPCORE_LANE_INSERT_TABLE_BYTES
+#define WARPCORE_LANE_INSERT_TABLE_BYTES std::uint64_t(0)
 #endif
 inline std::atomic<std::uint64_t>& lane_table_bytes_setting() noexcept {
     static std::atomic<std::uint64_t> bytes{WARPCORE_LANE_TABLE_BYTES};
     return bytes;
 }
+inline std::atomic<std::uint64_t>& lane_insert_table_bytes_setting() noexcept {
+    static std::atomic<std::uint64_t> bytes{WARPCORE_LANE_INSERT_TABLE_BYTES};
+    return bytes;
+}
 WC_H std::uint64_t lane_table_bytes() noexcept { return lane_table_bytes_setting().load(std::memory_order_relaxed); }
 WC_H void set_lane_table_bytes(std::uint64_t bytes) noexcept {
     lane_table_bytes_setting().store(bytes, std::memory_order_relaxed);
+}
+WC_H std::uint64_t lane_insert_table_bytes() noexcept {
+    return lane_insert_table_bytes_setting().load(std::memory_order_relaxed);
+}
+WC_H void set_lane_insert_table_bytes(std::uint64_t bytes) noexcept {
+    lane_insert_table_bytes_setting().store(bytes, std::memory_order_relaxed);
 }
 
 // Grid of a lane-per-key launch: keys_per_lane() keys per lane, at most 8

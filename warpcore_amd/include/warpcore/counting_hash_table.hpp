@@ -352,7 +352,7 @@ public:
                           hipStream_t stream, index_type probing_length,
                           typename StatusHandler::base_type* status_out) noexcept {
         if constexpr (detail::LaneWindow<typename view_type::slot_type, cg_size()>::usable) {
-            if (capacity() * sizeof(typename view_type::slot_type) <= detail::lane_table_bytes()) {
+            if (capacity() * sizeof(typename view_type::slot_type) <= detail::lane_insert_table_bytes()) {
                 kernels::insert_lanes<View, StatusHandler, detail::keys_per_lane(), HasValues>
                     <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, v,
                                                                                  probing_length, status_out);

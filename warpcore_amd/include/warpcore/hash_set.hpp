@@ -228,7 +228,7 @@ public:
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!is_initialized_ || num_in == 0) return;
         if constexpr (detail::LaneWindow<key_type, cg_size()>::usable) {
-            if (capacity() * sizeof(key_type) <= detail::lane_table_bytes()) {
+            if (capacity() * sizeof(key_type) <= detail::lane_insert_table_bytes()) {
                 kernels::insert_lanes<view_type, StatusHandler, detail::keys_per_lane(), false>
                     <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, nullptr, num_in, view(),
                                                                                  probing_length, status_out);

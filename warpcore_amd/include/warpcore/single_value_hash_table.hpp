@@ -369,7 +369,7 @@ public:
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!is_initialized_ || num_in == 0) return;
         if constexpr (detail::LaneWindow<slot_type, cg_size()>::usable) {
-            if (capacity() * sizeof(slot_type) <= detail::lane_table_bytes()) {
+            if (capacity() * sizeof(slot_type) <= detail::lane_insert_table_bytes()) {
                 kernels::insert_lanes<view_type, StatusHandler, detail::keys_per_lane(), true>
                     <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, view(),
                                                                                  probing_length, status_out);
