@@ -80,8 +80,10 @@ __global__ __launch_bounds__(256) void serve_prepare(const Desc* __restrict__ st
                                                      const std::uint32_t* __restrict__ staged_num, index_t max_reqs,
                                                      index_t item_keys, Desc* __restrict__ reqs,
                                                      index_t* __restrict__ first, index_t* __restrict__ state) {
-    const index_t num_raw = *reinterpret_cast<const volatile std::uint32_t*>(staged_num);
+    const index_t num_raw = reinterpret_cast<const volatile std::uint32_t*>(staged_num)[0];
     const index_t num = num_raw < max_reqs ? num_raw : max_reqs;
+    const index_t step_item_keys = reinterpret_cast<const volatile std::uint32_t*>(staged_num)[1];
+    if (step_item_keys != 0 && step_item_keys < item_keys) item_keys = step_item_keys;
     index_t carry = 0;
     for (index_t base = 0; base < num; base += blockDim.x) {
         const index_t r = base + threadIdx.x;
@@ -97,8 +99,9 @@ __global__ __launch_bounds__(256) void serve_prepare(const Desc* __restrict__ st
         carry += total;
     }
     if (threadIdx.x == 0) {
-        state[0] = num;     // live requests
-        state[1] = carry;   // work items
+        state[0] = num;         // live requests
+        state[1] = carry;       // work items
+        state[2] = item_keys;   // keys per work item of this step
     }
 }
 
@@ -136,6 +139,7 @@ __global__ __launch_bounds__(256) WC_SERVE_OCCUPANCY void serve_step(const servi
     const unsigned rank = lane_id() & (CG - 1);
     const index_t num = state[0] < max_reqs ? state[0] : max_reqs;
     const index_t total = state[1];
+    item_keys = state[2];
     for (index_t r = threadIdx.x; r < num; r += blockDim.x) first_lds[r] = first_item[r];
     __syncthreads();
     using H = status_handlers::ReturnStatus;
@@ -225,12 +229,12 @@ public:
         }
         std::vector<view_type> views;
         for (auto& t : tables_) views.push_back(t->view());
-        ok_ = alloc(&views_d_, views.size()) && alloc(&first_, cfg_.max_requests) && alloc(&state_, 2);
+        ok_ = alloc(&views_d_, views.size()) && alloc(&first_, cfg_.max_requests) && alloc(&state_, 3);
         for (int b = 0; b < 2 && ok_; ++b) {
             ok_ = alloc(&desc_d_[b], cfg_.max_requests) &&
                   hipHostMalloc(reinterpret_cast<void**>(&desc_h_[b]), sizeof(desc_type) * cfg_.max_requests) ==
                       hipSuccess &&
-                  hipHostMalloc(reinterpret_cast<void**>(&num_reqs_h_[b]), sizeof(std::uint32_t)) == hipSuccess;
+                  hipHostMalloc(reinterpret_cast<void**>(&num_reqs_h_[b]), 4 * sizeof(std::uint32_t)) == hipSuccess;
         }
         if (!ok_) return;
         // blocking streams: work the client queued on the default stream
@@ -332,7 +336,8 @@ public:
 private:
     struct Chunk {
         std::shared_ptr<Request> r;
-        bool last;   // the request's final chunk
+        bool last;      // the request's final chunk
+        index_t keys;   // keys of this chunk
     };
     using StepList = std::vector<Chunk>;
 
@@ -395,7 +400,7 @@ private:
             descs[step.size()] = d;
             r->issued += c;
             const bool last = r->issued == r->d.n;
-            step.push_back(Chunk{r, last});
+            step.push_back(Chunk{r, last, c});
             keys += c;
             mode[t] = op == Op::retrieve ? 0 : 1;
             if (!last) {   // the rest runs in a later step, ahead of this table's later requests
@@ -483,6 +488,20 @@ private:
         (void)hipGraphLaunch(graphs_[b].exec, stream_);
     }
 
+    // Keys per work item of a step: the smallest power of two from 8 that
+    // still gives every group about four items (at most cfg_.item_keys).
+    // A group takes ceil(items / groups) items, so small items keep the
+    // busiest group close to the average and a small step ends sooner;
+    // large steps keep large items, whose per-item setup is amortised.
+    index_t step_item_keys(const StepList& step) const noexcept {
+        index_t keys = 0;
+        for (const auto& c : step) keys += c.keys;
+        const index_t groups = index_t(step_grid_) * block_size() / CG;
+        index_t ik = 8;
+        while (ik < cfg_.item_keys && ik * groups * 4 < keys) ik *= 2;
+        return ik < cfg_.item_keys ? ik : cfg_.item_keys;
+    }
+
     struct InFlight {
         hipEvent_t ev;
         std::uint64_t step_no;
@@ -524,7 +543,8 @@ private:
             }
             if (!step.empty()) {
                 ++step_no;
-                *num_reqs_h_[b] = std::uint32_t(step.size());
+                num_reqs_h_[b][0] = std::uint32_t(step.size());
+                num_reqs_h_[b][1] = std::uint32_t(step_item_keys(step));
                 launch(b);
                 hipEvent_t ev = nullptr;
                 (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
@@ -559,7 +579,7 @@ private:
     desc_type* desc_d_[2] = {nullptr, nullptr};
     desc_type* desc_h_[2] = {nullptr, nullptr};
     std::uint32_t* num_reqs_h_[2] = {nullptr, nullptr};
-    index_t *first_ = nullptr, *state_ = nullptr;   // state: live requests, work items
+    index_t *first_ = nullptr, *state_ = nullptr;   // state: live requests, work items, keys per item
     unsigned step_grid_ = 0;
     hipStream_t stream_ = nullptr;
     hipStream_t side_ = nullptr;
