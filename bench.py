@@ -440,11 +440,12 @@ def main():
                                  "avg_kernel_ms": round(kt[other], 4), "traffic": read_traffic(other)}}
     # practical ceiling for random whole lines at the headline table's size
     lr = run_line_roofline(g, r64["table_bytes"])
-    tr = read_traffic("retrieve")
-    if tr:
-        lines_per_lookup = tr / LINE / n
-        lr["retrieve_lines_per_op"] = round(lines_per_lookup, 3)
-        lr["retrieve_frac"] = round(n / (kt["retrieve"] * 1e-3) * lines_per_lookup / 1e9 / lr["glines_s"], 4)
+    for op in ("retrieve", "insert"):
+        tr = read_traffic(op)
+        if tr:   # measured HBM bytes per op in 128-byte line equivalents (reads and write-backs)
+            lines_per_op = tr / LINE / n
+            lr[f"{op}_lines_per_op"] = round(lines_per_op, 3)
+            lr[f"{op}_frac"] = round(n / (kt[op] * 1e-3) * lines_per_op / 1e9 / lr["glines_s"], 4)
     log(f"[bench] line roofline: {lr}")
     cpu = None
     if not args.no_cpu_baseline:

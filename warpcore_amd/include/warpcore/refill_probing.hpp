@@ -37,7 +37,7 @@ namespace detail {
 #define WARPCORE_REFILL_KEYS 8
 #endif
 template<unsigned CG> constexpr int refill_keys() noexcept {
-    return CG < WARPCORE_REFILL_KEYS ? int(CG) : WARPCORE_REFILL_KEYS;
+    return CG < 2 ? 2 : WARPCORE_REFILL_KEYS;   // slots are not tied to lanes
 }
 #ifndef WARPCORE_REFILL_INSERT_KEYS
 #define WARPCORE_REFILL_INSERT_KEYS 4
