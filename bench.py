@@ -14,8 +14,12 @@ kernels run on, the roofline of the dominant kernel (algorithmic bytes: a
 retrieve reads one 128-byte line plus key and value, an insert reads and
 writes back one line plus key and value), the same workload for
 SingleValueHashTable<u32, u32> (cg 16) and CountingHashTable<u64> over 2^28
-Zipf(1.1) occurrences of 2^24 keys, and the OpenMP CPU table on a bounded
-sample as the reported CPU baseline.
+Zipf(1.1) occurrences of 2^24 keys, the plain 2^20-key u32 table (MALL
+resident, with the random-line rate at its own footprint), HashSet<u32>
+(2^27 keys), the pattern-blocked BloomFilter (2^30 bits, k 8, 2^27 keys,
+measured FPR), MultiValueHashTable<u32,u32> (2^26 pairs, two-pass
+retrieve), the batched small tables, the serving loop, and the OpenMP CPU
+table on a bounded sample as the reported CPU baseline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -53,12 +57,12 @@ def algorithmic_bytes(op: str, kb: int, vb: int) -> int:
     raise ValueError(op)
 
 
-def read_traffic(kernel_tag: str):
+def read_traffic(kernel_tag: str, log2n: int = 28):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
     WRITE_SIZE, scaled by a copy kernel of known size), if a summary for this
-    workload is committed under profiles/."""
+    workload (2^28 keys) is committed under profiles/."""
     f = REPO / "profiles" / "svht_u64_lf09_traffic.md"
-    if not f.exists():
+    if log2n != 28 or not f.exists():
         return None
     m = re.search(rf"^traffic_bytes_per_launch\.{kernel_tag}\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
     return float(m.group(1)) if m else None
@@ -334,6 +338,167 @@ def run_line_roofline(g: Gpu, table_bytes: int, lines: int = 1 << 28, reps: int 
             "ms": round(best, 3)}
 
 
+def _timed(g: Gpu, fn, reps: int, before=None):
+    """Min and mean over reps of fn() between HIP events on g.stream, after one
+    untimed call; before() (untimed, e.g. a table init) runs ahead of each."""
+    times = []
+    for r in range(reps + 1):
+        if before:
+            before()
+        e0, e1 = g.event(), g.event()
+        e0.record(g.stream)
+        fn()
+        e1.record(g.stream)
+        g.stream.synchronize()
+        if r:
+            times.append(e0.elapsed_time(e1))
+    return min(times), sum(times) / len(times)
+
+
+def _rate(n, ms, bytes_per_op):
+    """Gops/s and algorithmic GB/s of n ops in ms milliseconds."""
+    return {"gops": round(n / ms / 1e6, 3), "ms": round(ms, 4), "bytes_per_op": bytes_per_op,
+            "gbs": round(n * bytes_per_op / ms / 1e6, 1)}
+
+
+def run_small_svht(g: Gpu, load: float, reps: int = 20):
+    """The plain configuration: SingleValueHashTable<u32,u32>, 2^20 unique keys,
+    min capacity ceil(2^20 / load), lookups of all 2^20 keys plus 2^20 absent
+    keys in hashed order. The table (16.8 MB at load 0.8) stays in the MALL,
+    so the line rate at its own footprint is reported beside it."""
+    torch = g.torch
+    from warpcore_amd.tables import SingleValueHashTable
+    n = 1 << 20
+    keys, vals, absent = g.empty(n, 32), g.empty(n, 32), g.empty(n, 32)
+    queries, out = g.empty(2 * n, 32), g.empty(2 * n, 32)
+    with torch.cuda.stream(g.stream):
+        g.check(g.lib.wc_gen_unique_u32(g.p(keys), n, 0xC1, g.sp), "gen keys")
+        g.check(g.lib.wc_gen_uniform_u32(g.p(vals), n, 0xC1 + 1, g.sp), "gen values")
+        g.check(g.lib.wc_gen_unique_u32(g.p(absent), n, 0xC1 + n + 3, g.sp), "gen absent")
+        g.check(g.lib.wc_gen_interleave_u32(g.p(queries), g.p(keys), g.p(absent), n, 0xC1 + 2, g.sp), "mix")
+    t = SingleValueHashTable(math.ceil(n / load), 32, 32)
+    g.stream.synchronize()
+    ins, _ = _timed(g, lambda: t.insert(keys, vals, stream=g.stream), reps, before=lambda: t.init(stream=g.stream))
+    ret, _ = _timed(g, lambda: t.retrieve(queries, out, stream=g.stream), reps)
+    st = g.empty(2 * n, 32)
+    t.retrieve(queries, out, stream=g.stream, status_out=st)
+    g.stream.synchronize()
+    if int((st == 0).sum()) != n or t.size(stream=g.stream) != n:
+        raise RuntimeError("small-table result check failed")
+    lr = run_line_roofline(g, t.bytes_total(), lines=1 << 26)
+    res = {"capacity": t.capacity(), "table_bytes": t.bytes_total(), "cg": t.cg_size(),
+           "insert": _rate(n, ins, algorithmic_bytes("insert", 32, 32)),
+           "retrieve": _rate(2 * n, ret, algorithmic_bytes("retrieve", 32, 32)),
+           "line_rate_at_footprint_glines_s": lr["glines_s"]}
+    del t, keys, vals, absent, queries, out, st
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_hashset(g: Gpu, log2n: int = 27, reps: int = 3):
+    """HashSet<u32>: 2^27 keys at load factor 0.9 (151,011,248 slots, 604 MB),
+    insert, then contains of all keys and of 2^27 absent keys."""
+    torch = g.torch
+    from warpcore_amd.tables import HashSet
+    n = 1 << log2n
+    keys, absent = g.empty(n, 32), g.empty(n, 32)
+    with torch.cuda.stream(g.stream):
+        g.check(g.lib.wc_gen_unique_u32(g.p(keys), n, 0x3000, g.sp), "gen keys")
+        g.check(g.lib.wc_gen_unique_u32(g.p(absent), n, 0x3000 + n + 5, g.sp), "gen absent")
+    s = HashSet(math.ceil(n / 0.9), 32)
+    flags = torch.empty(n, dtype=torch.uint8, device="cuda")
+    g.stream.synchronize()
+    ins, _ = _timed(g, lambda: s.insert(keys, stream=g.stream), reps, before=lambda: s.init(stream=g.stream))
+    hit, _ = _timed(g, lambda: s.contains(keys, flags, stream=g.stream), reps)
+    ok = int(flags.sum()) == n
+    miss, _ = _timed(g, lambda: s.contains(absent, flags, stream=g.stream), reps)
+    ok = ok and int(flags.sum()) == 0 and s.size(stream=g.stream) == n
+    if not ok:
+        raise RuntimeError("hash set result check failed")
+    res = {"capacity": s.capacity(), "cg": s.cg_size(), "insert": _rate(n, ins, 2 * LINE + 4),
+           "contains_hits": _rate(n, hit, LINE + 4 + 1), "contains_misses": _rate(n, miss, LINE + 4 + 1)}
+    del s, keys, absent, flags
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_bloom(g: Gpu, log2_bits: int = 30, k: int = 8, log2n: int = 27, reps: int = 3):
+    """Pattern-blocked BloomFilter: 2^30 bits (128 MB), k = 8, 2^27 u64 inserts,
+    2^27 queries of which half are absent; measured false-positive rate."""
+    torch = g.torch
+    from warpcore_amd.tables import BloomFilter
+    n, half = 1 << log2n, 1 << (log2n - 1)
+    keys, absent, queries = g.empty(n, 64), g.empty(half, 64), g.empty(n, 64)
+    with torch.cuda.stream(g.stream):
+        g.check(g.lib.wc_gen_unique_u64(g.p(keys), n, 0x4000, g.sp), "gen keys")
+        g.check(g.lib.wc_gen_unique_u64(g.p(absent), half, 0x4000 + n + 9, g.sp), "gen absent")
+        g.check(g.lib.wc_gen_interleave_u64(g.p(queries), g.p(keys), g.p(absent), half, 0x4001, g.sp), "mix")
+    bf = BloomFilter(1 << log2_bits, k, 64)
+    flags = torch.empty(n, dtype=torch.uint8, device="cuda")
+    g.stream.synchronize()
+    ins, _ = _timed(g, lambda: bf.insert(keys, stream=g.stream), reps, before=lambda: bf.init(stream=g.stream))
+    qry, _ = _timed(g, lambda: bf.retrieve(queries, flags, stream=g.stream), reps)
+    bf.retrieve(keys, flags, stream=g.stream)
+    g.stream.synchronize()
+    if int(flags.sum()) != n:
+        raise RuntimeError("bloom filter returned a false negative")
+    bf.retrieve(absent, flags[:half], stream=g.stream)
+    g.stream.synchronize()
+    fpr = float(flags[:half].float().mean())
+    res = {"bytes": bf.bytes_total(), "cg": bf.cg_size(), "insert": _rate(n, ins, 2 * LINE + 8),
+           "query": _rate(n, qry, LINE + 8 + 1), "fpr_measured": round(fpr, 5), "fpr_model": round(bf.fpr(n), 5)}
+    del bf, keys, absent, queries, flags
+    torch.cuda.empty_cache()
+    return res
+
+
+def run_mvht(g: Gpu, log2n: int = 26, mean: int = 8, reps: int = 3):
+    """MultiValueHashTable<u32,u32>: 2^26 pairs, per-key multiplicity geometric
+    with mean 8 (about 2^23 keys), load factor 0.9 (75,505,624 slots); insert,
+    then the two-pass retrieve of every key (count, exclusive scan, gather)."""
+    import numpy as np
+    torch = g.torch
+    from warpcore_amd.tables import MultiValueHashTable
+    n = 1 << log2n
+    rng = np.random.default_rng(0xC4)
+    mult = rng.geometric(1 / mean, size=n // mean + 4096)
+    mult = mult[: np.searchsorted(np.cumsum(mult), n) + 1]
+    d = len(mult)
+    ukeys = g.empty(d, 32)
+    with torch.cuda.stream(g.stream):
+        g.check(g.lib.wc_gen_unique_u32(g.p(ukeys), d, 0x5000, g.sp), "gen keys")
+    g.stream.synchronize()
+    m = torch.from_numpy(mult).to("cuda")
+    pk = torch.repeat_interleave(ukeys, m)[:n]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0xC4)
+    pk = pk[torch.randperm(n, device="cuda", generator=gen)].contiguous()
+    pv = g.empty(n, 32)
+    with torch.cuda.stream(g.stream):
+        g.check(g.lib.wc_gen_uniform_u32(g.p(pv), n, 0xC5, g.sp), "gen values")
+    torch.cuda.synchronize()
+    t = MultiValueHashTable(math.ceil(n / 0.9), 32, 32)
+    g.stream.synchronize()
+    ins, _ = _timed(g, lambda: t.insert(pk, pv, stream=g.stream), reps, before=lambda: t.init(stream=g.stream))
+    nk = t.num_keys(stream=g.stream)
+    beg = torch.empty(nk, dtype=torch.int64, device="cuda")
+    end = torch.empty(nk, dtype=torch.int64, device="cuda")
+    vout = g.empty(n, 32)
+    got = []
+    ret, _ = _timed(g, lambda: got.append(t.retrieve(ukeys[:nk], beg, end, vout, stream=g.stream)), reps)
+    if nk != d or got[-1] != n:
+        raise RuntimeError(f"multi-value result check failed: keys {nk} of {d}, values {got[-1]} of {n}")
+    # two probe passes per key, offsets and scan (16 B per key), one value per pair out
+    per_key = 2 * LINE + 16 + 4
+    res = {"capacity": t.capacity(), "cg": t.cg_size(), "keys": nk, "pairs": n,
+           "insert": _rate(n, ins, 2 * LINE + 8),
+           "retrieve_pairs_gops": round(n / ret / 1e6, 3), "retrieve_ms": round(ret, 4),
+           "retrieve_gbs": round((nk * per_key + n * 4) / ret / 1e6, 1)}
+    del t, ukeys, m, pk, pv, beg, end, vout
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline(log2n: int = 24, reps: int = 3):
     """OpenMP table with the same hashing, probing and CAS protocol, on a
     2^log2n-key sample of the same workload (u64/u64, LF 0.9, 50 % hits)."""
@@ -370,9 +535,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2-keys", type=int, default=28)
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the u32, counting, batched-table and serving workloads")
+                    help="skip every workload but the headline u64 table")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched small tables")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving loop")
+    ap.add_argument("--no-structures", action="store_true",
+                    help="skip the 2^20-key u32 table, hash set, bloom filter and multi-value table")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-log2-keys", type=int, default=24)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
@@ -409,6 +576,14 @@ def main():
             log("[bench] batched tables ...")
             extras["batched_65536_u32u32"] = run_batched(g)
             log(f"[bench] batched: {extras['batched_65536_u32u32']}")
+        if not args.no_structures:
+            log("[bench] plain u32 table, hash set, bloom filter, multi-value table ...")
+            extras["svht_u32u32_2p20"] = {f"lf{lf}": run_small_svht(g, lf) for lf in (0.8, 0.9)}
+            extras["hashset_u32_2p27"] = run_hashset(g)
+            extras["bloom_2p30bits_k8"] = run_bloom(g)
+            extras["mvht_u32u32_2p26"] = run_mvht(g)
+            for k in ("svht_u32u32_2p20", "hashset_u32_2p27", "bloom_2p30bits_k8", "mvht_u32u32_2p26"):
+                log(f"[bench] {k}: {extras[k]}")
         if not args.no_serving:
             log("[bench] serving ...")
             extras["serving_64x1.56GB_u64u64"] = run_serving(g)
@@ -433,15 +608,16 @@ def main():
     achieved_other = per_op[other] * n / (kt[other] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": f"svht_u64u64_{dom}", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": read_traffic(dom),
+                "traffic": read_traffic(dom, args.log2_keys),
                 "bytes_per_op": per_op[dom], "ops_per_launch": n, "avg_kernel_ms": round(kt[dom], 4),
                 "other_kernel": {"kernel": f"svht_u64u64_{other}", "achieved": round(achieved_other, 1),
                                  "frac": round(achieved_other / HBM_PEAK_GBS, 4), "bytes_per_op": per_op[other],
-                                 "avg_kernel_ms": round(kt[other], 4), "traffic": read_traffic(other)}}
+                                 "avg_kernel_ms": round(kt[other], 4),
+                                 "traffic": read_traffic(other, args.log2_keys)}}
     # practical ceiling for random whole lines at the headline table's size
     lr = run_line_roofline(g, r64["table_bytes"])
     for op in ("retrieve", "insert"):
-        tr = read_traffic(op)
+        tr = read_traffic(op, args.log2_keys)
         if tr:   # measured HBM bytes per op in 128-byte line equivalents (reads and write-backs)
             lines_per_op = tr / LINE / n
             lr[f"{op}_lines_per_op"] = round(lines_per_op, 3)

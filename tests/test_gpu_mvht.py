@@ -5,9 +5,13 @@ import numpy as np
 import pytest
 import torch
 
-from datagen import dev_empty, dtype, from_dev, to_dev, unique_keys
+from datagen import dev_empty, dtype, from_dev, kernel_shape, to_dev, unique_keys
 
 pytestmark = pytest.mark.gpu
+
+# bulk inserts: one pair per group at a time ("groups") or several pairs in
+# flight per group ("refill", the default)
+INSERT_KERNELS = pytest.mark.parametrize("kernels", ["groups", "refill"])
 
 S_NONE, S_PLE, S_INVALID_KEY, S_DUP, S_NOT_FOUND = 0, 2, 8, 16, 32
 S_DRY_RUN, S_PHASE, S_MAXV = 1 << 9, 1 << 10, 1 << 11
@@ -35,18 +39,20 @@ def _expected(k, v):
     return {key: sorted(vals) for key, vals in out.items()}
 
 
+@INSERT_KERNELS
 @pytest.mark.parametrize("kb,cg,mean", [(32, 8, 8.0), (64, 8, 4.0), (32, 4, 2.0), (64, 16, 8.0), (32, 1, 3.0),
                                         (32, 64, 8.0)])
-def test_mvht_insert_retrieve_matches_oracle(cuda, oracle, kb, cg, mean):
+def test_mvht_insert_retrieve_matches_oracle(cuda, oracle, kb, cg, mean, kernels):
     from warpcore_amd.tables import MultiValueHashTable
-    print(f"[mvht] kb={kb} cg={cg} mean={mean}", flush=True)
+    print(f"[mvht] kb={kb} cg={cg} mean={mean} {kernels}", flush=True)
     keys, mult, k, v = _pairs(6000, kb, mean, seed=11 + cg)
     cap = int(len(k) / 0.8)
     t = MultiValueHashTable(cap, kb, kb, cg_size=cg)
     o = oracle.MVHT(cap, kb, kb, cg=cg)
     assert t.capacity() == o.capacity()
     st = dev_empty(len(k), 32, 0x77)
-    t.insert(to_dev(k), to_dev(v), status_out=st)
+    with kernel_shape(kernels):
+        t.insert(to_dev(k), to_dev(v), status_out=st)
     s_gpu = from_dev(st, np.uint32)
     s_ora = o.insert(k, v)
     # one `none` per distinct key, duplicates for the rest (which pair is first is unspecified)
@@ -101,13 +107,15 @@ def test_mvht_probe_order_parity_on_sequential_inserts(cuda, oracle, kb):
     np.testing.assert_array_equal(from_dev(vals, dtype(kb)), ov)
 
 
-def test_mvht_max_values_per_key(cuda, oracle):
+@INSERT_KERNELS
+def test_mvht_max_values_per_key(cuda, oracle, kernels):
     from warpcore_amd.tables import MultiValueHashTable
     keys, mult, k, v = _pairs(3000, 32, 6.0, seed=8)
     t = MultiValueHashTable(len(k) * 2, 32, 32, max_values_per_key=3)
     o = oracle.MVHT(len(k) * 2, 32, 32, cg=8, max_values=3)
     st = dev_empty(len(k), 32)
-    t.insert(to_dev(k), to_dev(v), status_out=st)
+    with kernel_shape(kernels):
+        t.insert(to_dev(k), to_dev(v), status_out=st)
     s_gpu = from_dev(st, np.uint32)
     s_ora = o.insert(k, v)
     stored = np.minimum(mult, 3)
@@ -174,22 +182,27 @@ def test_mvht_retrieve_all(cuda, kb):
                                                 (t.capacity() * 2 * kb / 8), rel=1e-5)
 
 
-def test_mvht_full_table_and_invalid_keys(cuda, oracle):
+@INSERT_KERNELS
+def test_mvht_full_table_and_invalid_keys(cuda, oracle, kernels):
     from warpcore_amd.tables import MultiValueHashTable
     t = MultiValueHashTable(256, 32, 32, cg_size=8)
     cap = t.capacity()
     k = np.repeat(unique_keys(cap // 4 + 10, 32, seed=4), 4)
     v = unique_keys(len(k), 32, seed=5)
     st = dev_empty(len(k), 32)
-    t.insert(to_dev(k), to_dev(v), status_out=st)
+    with kernel_shape(kernels):
+        t.insert(to_dev(k), to_dev(v), status_out=st)
     s = from_dev(st, np.uint32)
     assert t.size() == cap                           # every slot taken
     assert np.sum((s & S_PLE) != 0) == len(k) - cap  # the rest ran out of windows
+    # a pair out of windows after values of its key also says duplicate_key
+    assert set(np.unique(s)) <= {S_NONE, S_DUP, S_PLE, S_PLE | S_DUP}
     assert t.peek_status() & S_PLE
     bad = np.array([0, 0xFFFFFFFF], np.uint32)
     bst = dev_empty(2, 32)
     t2 = MultiValueHashTable(1000, 32, 32)
-    t2.insert(to_dev(bad), to_dev(np.array([1, 2], np.uint32)), status_out=bst)
+    with kernel_shape(kernels):
+        t2.insert(to_dev(bad), to_dev(np.array([1, 2], np.uint32)), status_out=bst)
     assert np.all(from_dev(bst, np.uint32) == S_INVALID_KEY)
     assert t2.size() == 0 and t2.num_keys() == 0
     assert t2.peek_status() == S_INVALID_KEY

@@ -27,7 +27,9 @@
 #include "defaults.hpp"
 #include "engine.hpp"
 #include "probing_schemes.hpp"
+#include "refill_probing.hpp"
 #include "scan.hpp"
+#include "single_value_hash_table.hpp"
 #include "status.hpp"
 #include "storage.hpp"
 #include "tile.hpp"
@@ -49,6 +51,21 @@ struct DeferredCount {
 struct ImmediateCount {
     index_t* word;
     WC_D void operator()(index_t n) const noexcept { if (word != nullptr && n) atomic_add(word, n); }
+};
+
+// Core of the bulk insert with several pairs in flight per group
+// (refill_probing.hpp): the slot hooks of the single-value table, the
+// per-key maximum, and the distinct-key counter, added to once per
+// wavefront at the end.
+template<class Key, class Value, Key EmptyKey, Key TombstoneKey, class ProbingScheme, class Slot>
+struct MultiValueInsertCore : SingleValueView<Key, Value, EmptyKey, TombstoneKey, ProbingScheme, Slot> {
+    static constexpr bool multi_value = true;
+    index_t max_values;
+    index_t* num_keys;
+    WC_D void add_keys(index_t n) const noexcept {
+        const index_t all = tile<64>::reduce_add(n);
+        if (lane_id() == 0 && all != 0 && num_keys != nullptr) atomic_add(num_keys, all);
+    }
 };
 
 template<class Key, class Value, Key EmptyKey, Key TombstoneKey, class ProbingScheme, class Slot>
@@ -357,6 +374,7 @@ public:
     using probing_scheme_type = ProbingScheme;
     using slot_type = typename TableStorage::slot_type;
     using view_type = detail::MultiValueView<Key, Value, EmptyKey, TombstoneKey, ProbingScheme, slot_type>;
+    using insert_core_type = detail::MultiValueInsertCore<Key, Value, EmptyKey, TombstoneKey, ProbingScheme, slot_type>;
 
     WC_HD static constexpr key_type empty_key() noexcept { return EmptyKey; }
     WC_HD static constexpr key_type tombstone_key() noexcept { return TombstoneKey; }
@@ -451,6 +469,16 @@ public:
                      typename StatusHandler::base_type* status_out = nullptr) noexcept {
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!is_initialized_ || num_in == 0) return;
+        if constexpr (detail::fixed_step_scheme<ProbingScheme>::value) {
+            if (detail::refill_inserts()) {   // several pairs in flight per group
+                const insert_core_type core{{table_.data(), status_, geo_, seed_, is_initialized_},
+                                            max_values_per_key_, num_keys_};
+                detail::launch_insert_refill<insert_core_type, StatusHandler,
+                                             detail::refill_insert_keys<unsigned(cg_size())>(), true>(
+                    keys_in, values_in, num_in, core, probing_length, status_out, stream);
+                return;
+            }
+        }
         kernels::mv_insert<view_type, StatusHandler>
             <<<detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(keys_in, values_in, num_in, view(),
                                                                                 probing_length, status_out);

@@ -46,6 +46,13 @@ template<unsigned CG> constexpr int refill_insert_keys() noexcept {
     return CG < WARPCORE_REFILL_INSERT_KEYS ? int(CG) : WARPCORE_REFILL_INSERT_KEYS;
 }
 
+// Cores of multi-value tables (static constexpr bool multi_value = true)
+// change the insert protocol: every pair takes a new slot, the key's pairs
+// met on the way are counted (multi_value_hash_table.cuh:200-289).
+template<class C, class = void> struct is_multi_value_core : std::false_type {};
+template<class C>
+struct is_multi_value_core<C, std::void_t<decltype(C::multi_value)>> : std::bool_constant<C::multi_value> {};
+
 // One staged key: the key, its first window and its window step.
 template<class K, class W>
 struct alignas(16) RefillEntry {
@@ -220,13 +227,21 @@ struct alignas(16) RefillInsertEntry {
 // then moves every slot on; a finished slot writes its result (value store
 // or counter update through finish_insert, then the status) and is
 // refilled the next round. `Core` provides the hooks of insert_batched.
+//
+// Multi-value cores: a window's lanes holding the key add to the key's
+// count instead of ending the insert, a claiming CAS that returns the key
+// counts one more and moves to the next EMPTY lane, and the insert ends
+// when a CAS claims a slot (none if no value of the key was met, else
+// duplicate_key; not joined) or when the count reaches core.max_values
+// (duplicate_key + max_values_for_key_reached, joined). A claim with no
+// earlier value adds one to *new_keys.
 template<class Core, class StatusHandler, int U, class W, bool HasValues>
 WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_in,
                               const typename Core::value_type* __restrict__ values_in, index_t first, index_t end,
                               const Core& core, index_t budget,
                               typename StatusHandler::base_type* __restrict__ status_out,
                               RefillInsertEntry<typename Core::key_type, W, typename Core::value_type>* stage,
-                              DeferredJoin& join) {
+                              DeferredJoin& join, index_t* new_keys = nullptr) {
     using K = typename Core::key_type;
     using V = typename Core::value_type;
     using Probe = typename Core::insert_probe_type;
@@ -267,8 +282,11 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
     Bits emp[U];            // group bits: EMPTY lanes of the current window not tried yet
     Probe seen[keep_seen ? U : 1];   // this lane's slot as read in the current window
     std::uint64_t live[U], claim[U];
+    constexpr bool multi = is_multi_value_core<Core>::value;
+    index_t count[multi ? U : 1];   // values of the key met so far (the same in every lane of the group)
 #pragma unroll
     for (int u = 0; u < U; ++u) {
+        if constexpr (multi) count[u] = 0;
         key[u] = K{};
         val[u] = V{};
         w[u] = 0;
@@ -280,7 +298,9 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
         live[u] = 0;
         claim[u] = 0;
     }
-    std::uint64_t dups = 0, exhausted = 0;   // groups that saw a duplicate / ran out of windows
+    std::uint64_t dups = 0, exhausted = 0;   // groups that saw a duplicate (multi-value: reached the
+                                             // maximum) / ran out of windows
+    std::uint64_t exhausted_dup = 0;         // multi-value: ran out of windows after values of the key
     index_t next = 0;
     std::uint64_t avail = 0;
     std::uint32_t stage_pos = 0;
@@ -340,6 +360,7 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
                 val[u] = e.value;
                 pos[u] = stage_pos + j;
                 rnd[u] = 0;
+                if constexpr (multi) count[u] = 0;
             }
             const std::uint64_t t = ballot(take);
             live[u] |= t;
@@ -372,10 +393,17 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
         for (int u = 0; u < U; ++u) {
             const std::uint64_t loading = live[u] & ~claim[u];
             const std::uint64_t claiming = live[u] & claim[u];
-            // loading slots: a hit is a duplicate, EMPTY lanes start the claim
+            // loading slots: a hit is a duplicate (multi-value: counted; the
+            // maximum reached ends the insert), EMPTY lanes start the claim
             const K tk = Core::probe_key(fresh[u]);
             const std::uint64_t hitl = ballot(tk == key[u]) & loading;
-            const std::uint64_t ghit = spread(hitl);
+            std::uint64_t ghit;
+            if constexpr (multi) {
+                count[u] += index_t(__builtin_popcountll(group_bits(hitl)));
+                ghit = spread(ballot(count[u] >= core.max_values) & loading);
+            } else {
+                ghit = spread(hitl);
+            }
             const std::uint64_t em = ballot(Core::is_empty_key(tk));
             const std::uint64_t gemp = spread(em & loading) & ~ghit;
             if constexpr (keep_seen)
@@ -394,7 +422,11 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
             }
             const int code = __builtin_amdgcn_ds_bpermute(int((shift + leader) << 2), outcome);
             const std::uint64_t won = ballot(code == 1) & claiming;
-            const std::uint64_t dupc = ballot(code == 2) & claiming;
+            std::uint64_t dupc = ballot(code == 2) & claiming;
+            if constexpr (multi) {   // the key won this slot first: one more value, then the next EMPTY lane
+                if (mine(dupc)) count[u] += 1;
+                dupc &= ballot(count[u] >= core.max_values);
+            }
             std::uint64_t retry = 0;   // only slots whose window read carries value bits
             if constexpr (keep_seen) retry = ballot(code == 3) & claiming;
             if (mine(gemp)) emp[u] = Bits(group_bits(em));
@@ -407,17 +439,33 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
             const std::uint64_t done = dup | won;
             dups |= dup;
             if (rank == 0 && mine(done)) {
-                const unsigned at_lane = mine(ghit) ? unsigned(ffs64(group_bits(hitl)) - 1) : leader;
-                Status st = mine(dup) ? Status::duplicate_key() : Status::none();
-                core.finish_insert(index_t(w[u]) * CG + at_lane, val[u], mine(won) && Core::publishes_value, st,
-                                   join);
-                StatusHandler::handle(st, status_out, first + pos[u]);
+                if constexpr (multi) {
+                    Status st = mine(dup) ? Status::duplicate_key() + Status::max_values_for_key_reached()
+                                          : (count[u] > 0 ? Status::duplicate_key() : Status::none());
+                    if (mine(won)) {
+                        core.finish_insert(index_t(w[u]) * CG + leader, val[u], Core::publishes_value, st, join);
+                        if (count[u] == 0 && new_keys != nullptr) *new_keys += 1;
+                    }
+                    StatusHandler::handle(st, status_out, first + pos[u]);
+                } else {
+                    const unsigned at_lane = mine(ghit) ? unsigned(ffs64(group_bits(hitl)) - 1) : leader;
+                    Status st = mine(dup) ? Status::duplicate_key() : Status::none();
+                    core.finish_insert(index_t(w[u]) * CG + at_lane, val[u], mine(won) && Core::publishes_value, st,
+                                       join);
+                    StatusHandler::handle(st, status_out, first + pos[u]);
+                }
             }
             // slots moving to the next window, or out of windows
             const std::uint64_t adv = (loading & ~ghit & ~gemp) | drained;
             const std::uint64_t last = adv & ballot(rnd[u] + 1 >= budget);
             exhausted |= last;
-            if (rank == 0 && mine(last)) StatusHandler::handle(Status::probing_length_exceeded(), status_out, first + pos[u]);
+            Status ple = Status::probing_length_exceeded();
+            if constexpr (multi) {
+                const std::uint64_t met = last & ballot(count[u] > 0);
+                exhausted_dup |= met;
+                if (mine(met)) ple = ple + Status::duplicate_key();
+            }
+            if (rank == 0 && mine(last)) StatusHandler::handle(ple, status_out, first + pos[u]);
             const std::uint64_t move = adv & ~last;
             const W nw = w[u] + step[u];
             if (mine(move)) {
@@ -428,7 +476,12 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
             live[u] &= ~(done | last);
         }
     }
-    if (mine(dups)) join(Status::duplicate_key());
+    if constexpr (multi) {
+        if (mine(dups)) join(Status::duplicate_key() + Status::max_values_for_key_reached());
+        if (mine(exhausted_dup)) join(Status::duplicate_key());
+    } else {
+        if (mine(dups)) join(Status::duplicate_key());
+    }
     if (mine(exhausted)) join(Status::probing_length_exceeded());
 }
 
@@ -483,10 +536,12 @@ __global__ __launch_bounds__(256) void insert_refill(const typename Core::key_ty
     const index_t first = gid * per < num_in ? gid * per : num_in;
     const index_t end = first + per < num_in ? first + per : num_in;
     const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, core.geometry().windows);
+    index_t new_keys = 0;
     detail::insert_refill_range<Core, StatusHandler, U, W, HasValues>(keys_in, values_in, first, end, core, budget,
                                                                       status_out, stage + (threadIdx.x & ~(CG - 1)),
-                                                                      join);
+                                                                      join, &new_keys);
     join.flush(core.status_word());
+    if constexpr (detail::is_multi_value_core<Core>::value) core.add_keys(new_keys);
 }
 
 }  // namespace kernels
