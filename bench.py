@@ -313,29 +313,39 @@ def run_serving(g: Gpu, num_tables: int = 64, total_bytes: float = 100e9, num_re
     return res
 
 
-def run_line_roofline(g: Gpu, table_bytes: int, lines: int = 1 << 28, reps: int = 3):
+def run_line_roofline(g: Gpu, table_bytes: int, lines: int = 1 << 28, reps: int = 3, claim: bool = False):
     """Rate at which random 128-byte lines of a table_bytes footprint are
     served (wc_util_gather_lines: 8 lanes per line, 8 lines in flight per
-    group), the practical ceiling of the probing kernels at that size."""
+    group), the practical ceiling of the probing kernels at that size; with
+    `claim`, also the rate of line reads each followed by a dependent
+    compare-and-swap on one of its words (wc_util_claim_lines), the
+    ceiling of one-window inserts."""
     torch = g.torch
     buf = torch.empty(table_bytes // 16 * 2, dtype=torch.int64, device="cuda")
     buf.fill_(7)
     sink = torch.zeros(1, dtype=torch.int32, device="cuda")
     g.stream.wait_stream(torch.cuda.current_stream())
-    best = float("inf")
-    for r in range(reps + 1):
-        e0, e1 = g.event(), g.event()
-        e0.record(g.stream)
-        g.check(g.lib.wc_util_gather_lines(g.p(buf), table_bytes, lines, 0x51 + r, g.p(sink), g.sp), "gather")
-        e1.record(g.stream)
-        g.stream.synchronize()
-        if r:
-            best = min(best, e0.elapsed_time(e1))
+    def best_of(fn, what):
+        best = float("inf")
+        for r in range(reps + 1):
+            e0, e1 = g.event(), g.event()
+            e0.record(g.stream)
+            g.check(fn(g.p(buf), table_bytes, lines, 0x51 + r, g.p(sink), g.sp), what)
+            e1.record(g.stream)
+            g.stream.synchronize()
+            if r:
+                best = min(best, e0.elapsed_time(e1))
+        return best
+
+    best = best_of(g.lib.wc_util_gather_lines, "gather")
+    res = {"glines_s": round(lines / (best * 1e-3) / 1e9, 2), "gbs": round(lines / (best * 1e-3) / 1e9 * LINE, 1),
+           "footprint_bytes": table_bytes, "ms": round(best, 3)}
+    if claim:   # each line read followed by a dependent CAS on one of its words: the insert's pattern
+        cbest = best_of(g.lib.wc_util_claim_lines, "claim")
+        res["claim_glines_s"] = round(lines / (cbest * 1e-3) / 1e9, 2)
     del buf, sink
     torch.cuda.empty_cache()
-    glines = lines / (best * 1e-3) / 1e9
-    return {"glines_s": round(glines, 2), "gbs": round(glines * LINE, 1), "footprint_bytes": table_bytes,
-            "ms": round(best, 3)}
+    return res
 
 
 def _timed(g: Gpu, fn, reps: int, before=None):
@@ -385,11 +395,12 @@ def run_small_svht(g: Gpu, load: float, reps: int = 20):
     g.stream.synchronize()
     if int((st == 0).sum()) != n or t.size(stream=g.stream) != n:
         raise RuntimeError("small-table result check failed")
-    lr = run_line_roofline(g, t.bytes_total(), lines=1 << 26)
+    lr = run_line_roofline(g, t.bytes_total(), lines=1 << 26, claim=True)
     res = {"capacity": t.capacity(), "table_bytes": t.bytes_total(), "cg": t.cg_size(),
            "insert": _rate(n, ins, algorithmic_bytes("insert", 32, 32)),
            "retrieve": _rate(2 * n, ret, algorithmic_bytes("retrieve", 32, 32)),
-           "line_rate_at_footprint_glines_s": lr["glines_s"]}
+           "line_rate_at_footprint_glines_s": lr["glines_s"],
+           "claim_rate_at_footprint_glines_s": lr["claim_glines_s"]}
     del t, keys, vals, absent, queries, out, st
     torch.cuda.empty_cache()
     return res
@@ -615,7 +626,9 @@ def main():
                                  "avg_kernel_ms": round(kt[other], 4),
                                  "traffic": read_traffic(other, args.log2_keys)}}
     # practical ceiling for random whole lines at the headline table's size
-    lr = run_line_roofline(g, r64["table_bytes"])
+    lr = run_line_roofline(g, r64["table_bytes"], claim=True)
+    # inserts against the read-then-claim rate (one window and one CAS per key at best)
+    lr["insert_vs_claim"] = round(n / (kt["insert"] * 1e-3) / 1e9 / lr["claim_glines_s"], 4)
     for op in ("retrieve", "insert"):
         tr = read_traffic(op, args.log2_keys)
         if tr:   # measured HBM bytes per op in 128-byte line equivalents (reads and write-backs)

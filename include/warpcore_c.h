@@ -582,6 +582,13 @@ WC_API int wc_util_copy(void* dst, const void* src, uint64_t bytes, void* stream
  * footprint, the practical ceiling of the probing kernels. `sink` is a
  * device word the kernel may write. No reference counterpart. */
 WC_API int wc_util_gather_lines(const void* table, uint64_t table_bytes, uint64_t lines, uint64_t seed, uint32_t* sink, void* stream);
+/* As wc_util_gather_lines, and then one lane per line replaces the 8 bytes
+ * it read by their value plus one with a compare-and-swap that returns the
+ * old word (the lane is chosen by the data read, so the CAS depends on the
+ * line): the rate of "read a random line, claim a word of it", the memory
+ * pattern of an insert without the probing. Writes into `table`. No
+ * reference counterpart. */
+WC_API int wc_util_claim_lines(void* table, uint64_t table_bytes, uint64_t lines, uint64_t seed, uint32_t* sink, void* stream);
 
 /* Capacity rule detail::get_valid_capacity (include/warpcore/base.cuh:27-34)
  * and the unique-key map, evaluated on the host. */
