@@ -9,9 +9,9 @@ from datagen import dev_empty, dtype, from_dev, kernel_shape, to_dev, unique_key
 
 pytestmark = pytest.mark.gpu
 
-# bulk inserts: one pair per group at a time ("groups") or several pairs in
-# flight per group ("refill", the default)
-INSERT_KERNELS = pytest.mark.parametrize("kernels", ["groups", "refill"])
+# bulk inserts and lookups: one key per group at a time ("groups") or
+# several keys in flight per group ("refill", the default)
+BULK_KERNELS = pytest.mark.parametrize("kernels", ["groups", "refill"])
 
 S_NONE, S_PLE, S_INVALID_KEY, S_DUP, S_NOT_FOUND = 0, 2, 8, 16, 32
 S_DRY_RUN, S_PHASE, S_MAXV = 1 << 9, 1 << 10, 1 << 11
@@ -39,7 +39,7 @@ def _expected(k, v):
     return {key: sorted(vals) for key, vals in out.items()}
 
 
-@INSERT_KERNELS
+@BULK_KERNELS
 @pytest.mark.parametrize("kb,cg,mean", [(32, 8, 8.0), (64, 8, 4.0), (32, 4, 2.0), (64, 16, 8.0), (32, 1, 3.0),
                                         (32, 64, 8.0)])
 def test_mvht_insert_retrieve_matches_oracle(cuda, oracle, kb, cg, mean, kernels):
@@ -53,39 +53,39 @@ def test_mvht_insert_retrieve_matches_oracle(cuda, oracle, kb, cg, mean, kernels
     st = dev_empty(len(k), 32, 0x77)
     with kernel_shape(kernels):
         t.insert(to_dev(k), to_dev(v), status_out=st)
-    s_gpu = from_dev(st, np.uint32)
-    s_ora = o.insert(k, v)
-    # one `none` per distinct key, duplicates for the rest (which pair is first is unspecified)
-    assert set(np.unique(s_gpu)) <= {S_NONE, S_DUP}
-    assert np.sum(s_gpu == S_NONE) == np.sum(s_ora == S_NONE) == len(keys)
-    assert len(np.unique(k[s_gpu == S_NONE])) == len(keys)
-    assert t.num_keys() == o.num_keys() == len(keys)
-    assert t.size() == o.size() == len(k)
-    assert t.peek_status() == o.status() == S_NONE
-    absent = unique_keys(500, kb, seed=99, exclude=keys)
-    q = np.concatenate([keys, absent])
-    rng = np.random.default_rng(5)
-    q = q[rng.permutation(len(q))]
-    n = len(q)
-    begin, end = dev_empty(n, 64), dev_empty(n, 64)
-    vals = dev_empty(len(k) + 8, kb, 0xAB)
-    qst = dev_empty(n, 32, 0x77)
-    total = t.retrieve(to_dev(q), begin, end, vals, status_out=qst)
-    ob, oe, ov, ost = o.retrieve(q)
-    assert total == len(k) == len(ov)
-    b, e = from_dev(begin, np.uint64), from_dev(end, np.uint64)
-    np.testing.assert_array_equal(b, ob)        # offsets depend only on counts
-    np.testing.assert_array_equal(e, oe)
-    np.testing.assert_array_equal(from_dev(qst, np.uint32), ost)
-    got = from_dev(vals, dtype(kb))
-    assert np.all(got[len(k):] == np.array(0xAB, dtype(kb)))   # nothing written past the total
-    seg = _segments(b, e, got, q)
-    exp = _expected(k, v)
-    for key in keys.tolist():
-        assert seg[key] == exp[key]
-    for key in absent.tolist():
-        assert seg[key] == []
-    assert t.peek_status() == S_NOT_FOUND
+        s_gpu = from_dev(st, np.uint32)
+        s_ora = o.insert(k, v)
+        # one `none` per distinct key, duplicates for the rest (which pair is first is unspecified)
+        assert set(np.unique(s_gpu)) <= {S_NONE, S_DUP}
+        assert np.sum(s_gpu == S_NONE) == np.sum(s_ora == S_NONE) == len(keys)
+        assert len(np.unique(k[s_gpu == S_NONE])) == len(keys)
+        assert t.num_keys() == o.num_keys() == len(keys)
+        assert t.size() == o.size() == len(k)
+        assert t.peek_status() == o.status() == S_NONE
+        absent = unique_keys(500, kb, seed=99, exclude=keys)
+        q = np.concatenate([keys, absent])
+        rng = np.random.default_rng(5)
+        q = q[rng.permutation(len(q))]
+        n = len(q)
+        begin, end = dev_empty(n, 64), dev_empty(n, 64)
+        vals = dev_empty(len(k) + 8, kb, 0xAB)
+        qst = dev_empty(n, 32, 0x77)
+        total = t.retrieve(to_dev(q), begin, end, vals, status_out=qst)
+        ob, oe, ov, ost = o.retrieve(q)
+        assert total == len(k) == len(ov)
+        b, e = from_dev(begin, np.uint64), from_dev(end, np.uint64)
+        np.testing.assert_array_equal(b, ob)        # offsets depend only on counts
+        np.testing.assert_array_equal(e, oe)
+        np.testing.assert_array_equal(from_dev(qst, np.uint32), ost)
+        got = from_dev(vals, dtype(kb))
+        assert np.all(got[len(k):] == np.array(0xAB, dtype(kb)))   # nothing written past the total
+        seg = _segments(b, e, got, q)
+        exp = _expected(k, v)
+        for key in keys.tolist():
+            assert seg[key] == exp[key]
+        for key in absent.tolist():
+            assert seg[key] == []
+        assert t.peek_status() == S_NOT_FOUND
 
 
 @pytest.mark.parametrize("kb", [32, 64])
@@ -107,7 +107,7 @@ def test_mvht_probe_order_parity_on_sequential_inserts(cuda, oracle, kb):
     np.testing.assert_array_equal(from_dev(vals, dtype(kb)), ov)
 
 
-@INSERT_KERNELS
+@BULK_KERNELS
 def test_mvht_max_values_per_key(cuda, oracle, kernels):
     from warpcore_amd.tables import MultiValueHashTable
     keys, mult, k, v = _pairs(3000, 32, 6.0, seed=8)
@@ -116,24 +116,24 @@ def test_mvht_max_values_per_key(cuda, oracle, kernels):
     st = dev_empty(len(k), 32)
     with kernel_shape(kernels):
         t.insert(to_dev(k), to_dev(v), status_out=st)
-    s_gpu = from_dev(st, np.uint32)
-    s_ora = o.insert(k, v)
-    stored = np.minimum(mult, 3)
-    assert t.size() == o.size() == int(stored.sum())
-    assert t.num_keys() == len(keys)
-    for code in (S_NONE, S_DUP, S_DUP | S_MAXV):
-        assert np.sum(s_gpu == code) == np.sum(s_ora == code)
-    assert t.peek_status() == o.status() == S_DUP | S_MAXV
-    per = dev_empty(len(keys), 64)
-    assert t.num_values(to_dev(keys), per) == int(stored.sum())
-    np.testing.assert_array_equal(from_dev(per, np.uint64), stored.astype(np.uint64))
-    # each kept value is one of the key's inserted values
-    begin, end, vals = dev_empty(len(keys), 64), dev_empty(len(keys), 64), dev_empty(int(stored.sum()), 32)
-    t.retrieve(to_dev(keys), begin, end, vals)
-    seg = _segments(from_dev(begin, np.uint64), from_dev(end, np.uint64), from_dev(vals, np.uint32), keys)
-    exp = _expected(k, v)
-    for key in keys.tolist():
-        assert set(seg[key]) <= set(exp[key]) and len(seg[key]) == min(len(exp[key]), 3)
+        s_gpu = from_dev(st, np.uint32)
+        s_ora = o.insert(k, v)
+        stored = np.minimum(mult, 3)
+        assert t.size() == o.size() == int(stored.sum())
+        assert t.num_keys() == len(keys)
+        for code in (S_NONE, S_DUP, S_DUP | S_MAXV):
+            assert np.sum(s_gpu == code) == np.sum(s_ora == code)
+        assert t.peek_status() == o.status() == S_DUP | S_MAXV
+        per = dev_empty(len(keys), 64)
+        assert t.num_values(to_dev(keys), per) == int(stored.sum())
+        np.testing.assert_array_equal(from_dev(per, np.uint64), stored.astype(np.uint64))
+        # each kept value is one of the key's inserted values
+        begin, end, vals = dev_empty(len(keys), 64), dev_empty(len(keys), 64), dev_empty(int(stored.sum()), 32)
+        t.retrieve(to_dev(keys), begin, end, vals)
+        seg = _segments(from_dev(begin, np.uint64), from_dev(end, np.uint64), from_dev(vals, np.uint32), keys)
+        exp = _expected(k, v)
+        for key in keys.tolist():
+            assert set(seg[key]) <= set(exp[key]) and len(seg[key]) == min(len(exp[key]), 3)
 
 
 def test_mvht_dry_run_and_num_values(cuda, oracle):
@@ -182,7 +182,7 @@ def test_mvht_retrieve_all(cuda, kb):
                                                 (t.capacity() * 2 * kb / 8), rel=1e-5)
 
 
-@INSERT_KERNELS
+@BULK_KERNELS
 def test_mvht_full_table_and_invalid_keys(cuda, oracle, kernels):
     from warpcore_amd.tables import MultiValueHashTable
     t = MultiValueHashTable(256, 32, 32, cg_size=8)

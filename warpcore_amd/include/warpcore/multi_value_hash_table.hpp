@@ -68,6 +68,54 @@ struct MultiValueInsertCore : SingleValueView<Key, Value, EmptyKey, TombstoneKey
     }
 };
 
+// Cores of the two lookup passes with several keys in flight per group
+// (refill_probing.hpp): the count pass (Gather = false) writes each key's
+// value count and adds them up into *total; the gather pass writes the
+// j-th value of key i to values_out[begin_in[i] + j] while j is below the
+// key's room end_in[i] - begin_in[i], and flags invalid_phase_overlap when
+// the key's count differs from its room (gpu_engine.cuh:305-350, 466-504).
+template<class Key, class Value, Key EmptyKey, Key TombstoneKey, class ProbingScheme, class Slot, bool Gather>
+struct MultiValueLookupCore : SingleValueView<Key, Value, EmptyKey, TombstoneKey, ProbingScheme, Slot> {
+    static constexpr bool multi_value = true;
+    index_t max_values;
+    index_t* per_key_out;       // count pass, nullable
+    index_t* total;             // count pass
+    const index_t* begin_in;    // gather pass
+    const index_t* end_in;
+    Value* values_out;
+    WC_D void range_of(index_t i, index_t& base, index_t& room) const noexcept {
+        if constexpr (Gather) {
+            base = begin_in[i];
+            room = end_in[i] - base;
+        } else {
+            base = 0;
+            room = 0;
+        }
+    }
+    WC_D void emit(index_t base, index_t room, index_t j, const Value& v) const noexcept {
+        if constexpr (Gather)
+            if (j < room) values_out[base + j] = v;
+    }
+    template<class Join>
+    WC_D void finish(index_t i, index_t count, index_t room, Status& st, Join& join, index_t* tally) const noexcept {
+        if constexpr (Gather) {
+            if (count != room) {
+                join(Status::invalid_phase_overlap());
+                st += Status::invalid_phase_overlap();
+            }
+        } else {
+            if (per_key_out != nullptr) per_key_out[i] = count;
+            *tally += count;
+        }
+    }
+    WC_D void add_tally(index_t n) const noexcept {
+        if constexpr (!Gather) {
+            const index_t all = tile<64>::reduce_add(n);
+            if (lane_id() == 0 && all != 0 && total != nullptr) atomic_add(total, all);
+        }
+    }
+};
+
 template<class Key, class Value, Key EmptyKey, Key TombstoneKey, class ProbingScheme, class Slot>
 struct MultiValueView {
     using key_type = Key;
@@ -375,6 +423,9 @@ public:
     using slot_type = typename TableStorage::slot_type;
     using view_type = detail::MultiValueView<Key, Value, EmptyKey, TombstoneKey, ProbingScheme, slot_type>;
     using insert_core_type = detail::MultiValueInsertCore<Key, Value, EmptyKey, TombstoneKey, ProbingScheme, slot_type>;
+    template<bool Gather>
+    using lookup_core_type =
+        detail::MultiValueLookupCore<Key, Value, EmptyKey, TombstoneKey, ProbingScheme, slot_type, Gather>;
 
     WC_HD static constexpr key_type empty_key() noexcept { return EmptyKey; }
     WC_HD static constexpr key_type tombstone_key() noexcept { return TombstoneKey; }
@@ -507,10 +558,22 @@ public:
             detail::device_scan(end_offsets_out, end_offsets_out, num_in, scan_parts_, true, stream);
             kernels::shift_offsets<index_type><<<detail::grid_for(num_in, 1, 8), block_size(), 0, stream>>>(
                 begin_offsets_out, end_offsets_out, num_in);
-            kernels::mv_retrieve<view_type, StatusHandler>
-                <<<detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(
-                    keys_in, num_in, begin_offsets_out, end_offsets_out, values_out, view(), probing_length,
-                    status_out);
+            bool gathered = false;
+            if constexpr (detail::fixed_step_scheme<ProbingScheme>::value) {
+                if (detail::refill_lookups()) {   // several keys in flight per group
+                    const lookup_core_type<true> core{{table_.data(), status_, geo_, seed_, is_initialized_},
+                                                      max_values_per_key_, nullptr, nullptr,
+                                                      begin_offsets_out, end_offsets_out, values_out};
+                    detail::launch_retrieve_refill<lookup_core_type<true>, StatusHandler, 4>(
+                        keys_in, num_in, nullptr, core, probing_length, status_out, stream);
+                    gathered = true;
+                }
+            }
+            if (!gathered)
+                kernels::mv_retrieve<view_type, StatusHandler>
+                    <<<detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(
+                        keys_in, num_in, begin_offsets_out, end_offsets_out, values_out, view(), probing_length,
+                        status_out);
         } else {
             if (status_out != nullptr)
                 kernels::handle_all<StatusHandler><<<detail::grid_for(num_in, 1, 8), block_size(), 0, stream>>>(
@@ -588,7 +651,18 @@ public:
         if (!is_initialized_) return;
         index_type* tmp = temp_.get();
         (void)hipMemsetAsync(tmp, 0, sizeof(index_type), stream);
-        if (num_in != 0)
+        bool counted = num_in == 0;
+        if constexpr (detail::fixed_step_scheme<ProbingScheme>::value) {
+            if (!counted && detail::refill_lookups()) {   // several keys in flight per group
+                const lookup_core_type<false> core{{table_.data(), status_, geo_, seed_, is_initialized_},
+                                                   max_values_per_key_, num_per_key_out, tmp,
+                                                   nullptr, nullptr, nullptr};
+                detail::launch_retrieve_refill<lookup_core_type<false>, StatusHandler, 4>(
+                    keys_in, num_in, nullptr, core, probing_length, status_out, stream);
+                counted = true;
+            }
+        }
+        if (!counted)
             kernels::mv_num_values<view_type, StatusHandler>
                 <<<detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(
                     keys_in, num_in, tmp, num_per_key_out, view(), probing_length, status_out);

@@ -64,11 +64,22 @@ struct alignas(16) RefillEntry {
 // wavefront call this together; each group passes its own range). W: window
 // index type (std::uint32_t below 2^31 windows). `stage` holds CG entries of
 // this group.
+//
+// Multi-value cores (multi_value_hash_table.cuh:605-661 per key): when a
+// slot takes key i it asks core.range_of(i, base, room) for the key's
+// output range (read while the first window is in flight); a hit does not
+// end the key, its lanes holding the key pass each value to
+// core.emit(base, room, j, value) (the j-th value of the key in probe
+// order); the key ends at a window with an EMPTY lane or once
+// core.max_values values were met, with none, or key_not_found (joined)
+// if it has no value, and core.finish(i, count, room, status, join,
+// *tally) sees the key's value count.
 template<class Core, class StatusHandler, int U, class W>
 WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys_in, index_t first, index_t end,
                                 typename Core::out_type* __restrict__ values_out, const Core& core, index_t budget,
                                 typename StatusHandler::base_type* __restrict__ status_out,
-                                RefillEntry<typename Core::key_type, W>* stage, DeferredJoin& join) {
+                                RefillEntry<typename Core::key_type, W>* stage, DeferredJoin& join,
+                                index_t* tally = nullptr) {
     using K = typename Core::key_type;
     using O = typename Core::out_type;
     using Slot = typename Core::probe_slot_type;
@@ -86,12 +97,14 @@ WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys
     auto mine = [](std::uint64_t m) -> bool { return __builtin_amdgcn_inverse_ballot_w64(m); };
     // lanes of every group that has a lane in m
     auto spread = [&](std::uint64_t m) -> std::uint64_t { return ballot((m & gm) != 0); };
+    auto group_bits = [&](std::uint64_t m) -> std::uint64_t { return CG == 64 ? m : ((m >> shift) & full); };
+    constexpr bool multi = is_multi_value_core<Core>::value;
 
     const probing_schemes::Geometry& geo = core.geometry();
     const W p = W(geo.windows);
     const Slot* __restrict__ lane_slots = core.slot_ptr() + rank;
     const index_t len = end > first ? end - first : 0;
-    O* __restrict__ out = values_out + first;
+    O* __restrict__ out = values_out != nullptr ? values_out + first : nullptr;
 
     if (!core.ready()) {   // uniform: every key reports not_initialized
         for (index_t i = rank; i < len; i += CG) StatusHandler::handle(Status::not_initialized(), status_out, first + i);
@@ -107,8 +120,15 @@ WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys
     W w[U], step[U];
     std::uint32_t pos[U], rnd[U];
     std::uint64_t live[U];
+    index_t count[multi ? U : 1];   // multi-value: values of the key met so far
+    index_t base[multi ? U : 1], room[multi ? U : 1];   // multi-value: the key's output range
 #pragma unroll
     for (int u = 0; u < U; ++u) {
+        if constexpr (multi) {
+            count[u] = 0;
+            base[u] = 0;
+            room[u] = 0;
+        }
         key[u] = K{};
         w[u] = 0;
         step[u] = 1;
@@ -167,6 +187,10 @@ WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys
                 step[u] = e.step;
                 pos[u] = stage_pos + j;
                 rnd[u] = 0;
+                if constexpr (multi) {
+                    count[u] = 0;
+                    core.range_of(first + pos[u], base[u], room[u]);
+                }
             }
             live[u] |= ballot(take);
             any_live |= live[u];
@@ -186,6 +210,32 @@ WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys
         for (int u = 0; u < U; ++u) {
             const K tk = Core::key_of(s[u]);
             const std::uint64_t hit = ballot(tk == key[u]) & live[u];
+            if constexpr (multi) {
+                const std::uint64_t gh = group_bits(hit);
+                if (mine(hit))
+                    core.emit(base[u], room[u],
+                              count[u] + index_t(__builtin_popcountll(gh & ((std::uint64_t(1) << rank) - 1))),
+                              Core::value_of(s[u]));
+                count[u] += index_t(__builtin_popcountll(gh));
+                const std::uint64_t stop =
+                    spread((ballot(Core::is_empty_key(tk)) | ballot(count[u] >= core.max_values)) & live[u]);
+                const std::uint64_t out_of_windows = live[u] & ~stop & ballot(rnd[u] + 1 >= budget);
+                const std::uint64_t missing = stop & ballot(count[u] == 0);
+                missed |= missing;
+                exhausted |= out_of_windows;
+                if (rank == 0 && mine(stop | out_of_windows)) {
+                    Status st = mine(out_of_windows) ? Status::probing_length_exceeded()
+                                                     : (mine(missing) ? Status::key_not_found() : Status::none());
+                    core.finish(first + pos[u], count[u], room[u], st, join, tally);
+                    StatusHandler::handle(st, status_out, first + pos[u]);
+                }
+                const std::uint64_t cont = live[u] & ~stop & ~out_of_windows;
+                const W nw = w[u] + step[u];
+                w[u] = mine(cont) ? (nw >= p ? W(nw - p) : nw) : w[u];
+                rnd[u] += 1;
+                live[u] = cont;
+                continue;
+            }
             const std::uint64_t stop = spread(hit | (ballot(Core::is_empty_key(tk)) & live[u]));
             const std::uint64_t ghit = spread(hit);
             const std::uint64_t out_of_windows = live[u] & ~stop & ballot(rnd[u] + 1 >= budget);
@@ -501,6 +551,15 @@ namespace kernels {
 #else
 #define WC_REFILL_OCCUPANCY
 #endif
+// The same for the refill insert kernel.
+#ifndef WARPCORE_REFILL_INSERT_WAVES
+#define WARPCORE_REFILL_INSERT_WAVES 0
+#endif
+#if WARPCORE_REFILL_INSERT_WAVES > 0
+#define WC_REFILL_INSERT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WARPCORE_REFILL_INSERT_WAVES)))
+#else
+#define WC_REFILL_INSERT_OCCUPANCY
+#endif
 
 // Bulk lookups: group g of the grid takes keys [g * per, (g + 1) * per).
 template<class Core, class StatusHandler, int U, class W>
@@ -517,13 +576,16 @@ __global__ __launch_bounds__(256) WC_REFILL_OCCUPANCY void retrieve_refill(const
     const index_t first = gid * per < num_in ? gid * per : num_in;
     const index_t end = first + per < num_in ? first + per : num_in;
     const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, core.geometry().windows);
+    index_t tally = 0;
     detail::retrieve_refill_range<Core, StatusHandler, U, W>(keys_in, first, end, values_out, core, budget,
-                                                             status_out, stage + (threadIdx.x & ~(CG - 1)), join);
+                                                             status_out, stage + (threadIdx.x & ~(CG - 1)), join,
+                                                             &tally);
     join.flush(core.status_word());
+    if constexpr (detail::is_multi_value_core<Core>::value) core.add_tally(tally);
 }
 
 template<class Core, class StatusHandler, int U, class W, bool HasValues>
-__global__ __launch_bounds__(256) void insert_refill(const typename Core::key_type* __restrict__ keys_in,
+__global__ __launch_bounds__(256) WC_REFILL_INSERT_OCCUPANCY void insert_refill(const typename Core::key_type* __restrict__ keys_in,
                                                      const typename Core::value_type* __restrict__ values_in,
                                                      index_t num_in, Core core, index_t probing_length,
                                                      typename StatusHandler::base_type* __restrict__ status_out) {
