@@ -286,7 +286,8 @@ def run_serving(g: Gpu, num_tables: int = 64, total_bytes: float = 100e9, num_re
     def one(mode, arrival, count=num_requests):
         srv = Server(caps, 64, 64, **(server_kw or {}))
         lat, mk = srv.drive(mode, ops[:count], tabs[:count], sizes[:count], arrival[:count], keys, values, out, st)
-        steps = srv.stats()["steps"] if mode == 0 else count
+        stats = srv.stats()
+        steps = (stats["steps"], stats["direct_steps"]) if mode == 0 else (count, 0)
         del srv
         torch.cuda.empty_cache()
         return lat, mk, steps
@@ -298,7 +299,8 @@ def run_serving(g: Gpu, num_tables: int = 64, total_bytes: float = 100e9, num_re
     lat_b, mk_b, _ = one(1, zero)
     lat_s, mk_s, steps_s = one(0, zero)
     res["throughput"] = {"server_gkeys_s": round(n / mk_s / 1e3, 3), "launch_per_request_gkeys_s":
-                         round(n / mk_b / 1e3, 3), "speedup": round(mk_b / mk_s, 3), "server_steps": steps_s}
+                         round(n / mk_b / 1e3, 3), "speedup": round(mk_b / mk_s, 3), "server_steps": steps_s[0],
+                         "server_direct_steps": steps_s[1]}
     rate = load * num_requests / mk_b          # requests per microsecond
     arrival = np.cumsum(rng.exponential(1.0 / rate, size=num_requests))
     lat_b, mk_b2, _ = one(1, arrival)
@@ -307,7 +309,7 @@ def run_serving(g: Gpu, num_tables: int = 64, total_bytes: float = 100e9, num_re
     res["latency"] = {"offered_load": load, "offered_gkeys_s": round(n / arrival[-1] / 1e3, 3),
                       "server_p50_us": pct(lat_s, 50), "server_p99_us": pct(lat_s, 99),
                       "launch_per_request_p50_us": pct(lat_b, 50), "launch_per_request_p99_us": pct(lat_b, 99),
-                      "server_steps": steps_s2}
+                      "server_steps": steps_s2[0], "server_direct_steps": steps_s2[1]}
     del keys, values, out, st
     torch.cuda.empty_cache()
     return res

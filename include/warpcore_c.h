@@ -490,19 +490,24 @@ WC_API uint32_t wc_batch_u64u64_cg_size(void* batch);
  * use_graph. A step holds at most max_step_keys keys; a larger request
  * runs in chunks over consecutive steps. Per table, requests run in
  * arrival order (a step holds a run of lookups or one insert/erase, or one
- * chunk of it, per table). An insert that does not fit its table's slot
- * budget (max_load * capacity, never given back) is refused: every key
- * reports out_of_memory and wait returns 0.
+ * chunk of it, per table). While fewer than 2^24 keys are pending (latency
+ * mode), a request of at least direct_keys keys runs whole as a step of its
+ * own on its table's bulk kernels (the kernels of one plain call). An
+ * insert that does not fit its table's slot budget (max_load * capacity,
+ * never given back) is refused: every key reports out_of_memory and wait
+ * returns 0.
  *   create       table_capacities: HOST array (minimum capacities);
  *                max_requests per step (0 = 1024), item_keys per work item
  *                (0 = 64), max_step_keys (0 = 2^24), max_load (<= 0: 0.9),
- *                blocks_per_cu (0 = 8)
+ *                blocks_per_cu (0 = 8), direct_keys (0 = 2^18, UINT64_MAX =
+ *                every step on the step kernel)
  *   submit       device buffers, kept valid until the request is done;
  *                returns the request id (-1 on a bad argument)
  *   wait / drain block until one / every request is done
  *   latency_us   arrival to completion of a done request
  *   step_of      the step a request ran in (1, 2, ...)
  *   stats        out3 = {requests done, steps, keys}
+ *   direct_steps steps run on a table's bulk kernels
  *   drive        replays num_requests requests (keys/values/outputs are
  *                slices of concatenated device arrays, sizes[i] each) at
  *                arrival_us[i] after the call starts; mode 0 through the
@@ -511,7 +516,7 @@ WC_API uint32_t wc_batch_u64u64_cg_size(void* batch);
  *                *makespan_us
  * ---------------------------------------------------------------------- */
 /* Server over SingleValueHashTable<uint64_t, uint64_t> */
-WC_API void* wc_server_u64u64_create(const uint64_t* table_capacities, uint64_t num_tables, uint64_t max_requests, uint64_t item_keys, uint64_t max_step_keys, float max_load, int use_graph, int blocks_per_cu);
+WC_API void* wc_server_u64u64_create(const uint64_t* table_capacities, uint64_t num_tables, uint64_t max_requests, uint64_t item_keys, uint64_t max_step_keys, float max_load, int use_graph, int blocks_per_cu, uint64_t direct_keys);
 WC_API void wc_server_u64u64_destroy(void* server);
 WC_API int64_t wc_server_u64u64_submit(void* server, uint32_t op, uint32_t table, const uint64_t* keys_in, const uint64_t* values_in, uint64_t* values_out, uint32_t* status_out, uint64_t num_in);
 WC_API int wc_server_u64u64_wait(void* server, int64_t id);
@@ -519,13 +524,14 @@ WC_API int wc_server_u64u64_drain(void* server);
 WC_API double wc_server_u64u64_latency_us(void* server, int64_t id);
 WC_API uint64_t wc_server_u64u64_step_of(void* server, int64_t id);
 WC_API int wc_server_u64u64_stats(void* server, uint64_t* out3);
+WC_API uint64_t wc_server_u64u64_direct_steps(void* server);
 WC_API uint64_t wc_server_u64u64_table_size(void* server, uint32_t table);
 WC_API uint64_t wc_server_u64u64_table_capacity(void* server, uint32_t table);
 WC_API uint32_t wc_server_u64u64_table_status(void* server, uint32_t table);
 WC_API int wc_server_u64u64_drive(void* server, int mode, const uint32_t* ops, const uint32_t* tables, const uint64_t* sizes, const double* arrival_us, uint64_t num_requests, const uint64_t* keys_in, const uint64_t* values_in, uint64_t* values_out, uint32_t* status_out, double* latency_us_out, double* makespan_us);
 
 /* Server over SingleValueHashTable<uint32_t, uint32_t> */
-WC_API void* wc_server_u32u32_create(const uint64_t* table_capacities, uint64_t num_tables, uint64_t max_requests, uint64_t item_keys, uint64_t max_step_keys, float max_load, int use_graph, int blocks_per_cu);
+WC_API void* wc_server_u32u32_create(const uint64_t* table_capacities, uint64_t num_tables, uint64_t max_requests, uint64_t item_keys, uint64_t max_step_keys, float max_load, int use_graph, int blocks_per_cu, uint64_t direct_keys);
 WC_API void wc_server_u32u32_destroy(void* server);
 WC_API int64_t wc_server_u32u32_submit(void* server, uint32_t op, uint32_t table, const uint32_t* keys_in, const uint32_t* values_in, uint32_t* values_out, uint32_t* status_out, uint64_t num_in);
 WC_API int wc_server_u32u32_wait(void* server, int64_t id);
@@ -533,6 +539,7 @@ WC_API int wc_server_u32u32_drain(void* server);
 WC_API double wc_server_u32u32_latency_us(void* server, int64_t id);
 WC_API uint64_t wc_server_u32u32_step_of(void* server, int64_t id);
 WC_API int wc_server_u32u32_stats(void* server, uint64_t* out3);
+WC_API uint64_t wc_server_u32u32_direct_steps(void* server);
 WC_API uint64_t wc_server_u32u32_table_size(void* server, uint32_t table);
 WC_API uint64_t wc_server_u32u32_table_capacity(void* server, uint32_t table);
 WC_API uint32_t wc_server_u32u32_table_status(void* server, uint32_t table);

@@ -64,16 +64,25 @@ def _run(mode, caps, ops, tables, sizes, arrival, keys, vals, **kw):
     return res
 
 
-@pytest.mark.parametrize("use_graph,max_step_keys", [(True, 0), (False, 0), (True, 5000)])
-def test_server_matches_one_launch_per_request(cuda, use_graph, max_step_keys):
+@pytest.mark.parametrize("use_graph,max_step_keys,direct_keys",
+                         [(True, 0, None), (False, 0, None), (True, 5000, None), (True, 0, 2048)])
+def test_server_matches_one_launch_per_request(cuda, use_graph, max_step_keys, direct_keys):
     """max_step_keys=5000: requests of up to 2^16 keys run in chunks over
-    several steps."""
+    several steps. direct_keys=2048: with a short backlog, requests of 2048
+    keys or more run alone on their table's bulk kernels, the rest in
+    steps of the step kernel."""
+    print(f"[serving] use_graph={use_graph} max_step_keys={max_step_keys} direct_keys={direct_keys}", flush=True)
     caps = np.full(6, 1 << 20, np.uint64)
     ops, tables, sizes, arrival, keys, vals = _stream(240, len(caps), seed=5)
     out_s, st_s, lat_s, mk_s, size_s, tst_s, stats = _run(0, caps, ops, tables, sizes, arrival, keys, vals,
-                                                          use_graph=use_graph, max_step_keys=max_step_keys)
+                                                          use_graph=use_graph, max_step_keys=max_step_keys,
+                                                          direct_keys=direct_keys)
     out_p, st_p, lat_p, mk_p, size_p, tst_p, _ = _run(1, caps, ops, tables, sizes, arrival, keys, vals)
     assert stats["requests"] == len(ops) and stats["keys"] == int(sizes.sum())
+    if direct_keys is None:
+        assert stats["direct_steps"] == 0
+    else:
+        assert 0 < stats["direct_steps"] <= int(np.sum(sizes >= direct_keys))
     if max_step_keys:
         assert stats["steps"] >= int(sizes.sum()) // max_step_keys
     else:

@@ -21,13 +21,14 @@ using server_t = serving::Server<K, V, CG>;
 
 template<class S>
 S* make(const std::uint64_t* caps, std::uint64_t n, std::uint64_t max_requests, std::uint64_t item_keys,
-        std::uint64_t max_step_keys, float max_load, int use_graph, int blocks_per_cu) {
+        std::uint64_t max_step_keys, float max_load, int use_graph, int blocks_per_cu, std::uint64_t direct_keys) {
     typename S::Config cfg;
     if (max_requests) cfg.max_requests = max_requests;
     if (item_keys) cfg.item_keys = item_keys;
     if (max_step_keys) cfg.max_step_keys = max_step_keys;
     if (max_load > 0.0f) cfg.max_load = max_load;
     if (blocks_per_cu > 0) cfg.blocks_per_cu = blocks_per_cu;
+    if (direct_keys) cfg.direct_keys = direct_keys == ~std::uint64_t(0) ? 0 : direct_keys;
     cfg.use_graph = use_graph != 0;
     S* s = new S(caps, n, cfg);
     if (!s->ok()) {
@@ -122,11 +123,11 @@ int drive(S* s, int mode, const std::uint32_t* ops, const std::uint32_t* tables,
     extern "C" void* wc_server_##NAME##_create(const uint64_t* table_capacities, uint64_t num_tables,             \
                                                uint64_t max_requests, uint64_t item_keys,                         \
                                                uint64_t max_step_keys, float max_load, int use_graph,             \
-                                               int blocks_per_cu) {                                               \
+                                               int blocks_per_cu, uint64_t direct_keys) {                         \
         if (!table_capacities || num_tables == 0) return nullptr;                                                 \
         try {                                                                                                     \
             return make<Srv_##NAME>(table_capacities, num_tables, max_requests, item_keys, max_step_keys,         \
-                                    max_load, use_graph, blocks_per_cu);                                          \
+                                    max_load, use_graph, blocks_per_cu, direct_keys);                             \
         } catch (...) { return nullptr; }                                                                         \
     }                                                                                                             \
     extern "C" void wc_server_##NAME##_destroy(void* h) { delete static_cast<Srv_##NAME*>(h); }                  \
@@ -161,6 +162,9 @@ int drive(S* s, int mode, const std::uint32_t* ops, const std::uint32_t* tables,
         out3[1] = s.steps;                                                                                        \
         out3[2] = s.keys;                                                                                         \
         return 0;                                                                                                 \
+    }                                                                                                             \
+    extern "C" uint64_t wc_server_##NAME##_direct_steps(void* h) {                                                \
+        return h ? static_cast<Srv_##NAME*>(h)->stats().direct_steps : 0;                                        \
     }                                                                                                             \
     extern "C" uint64_t wc_server_##NAME##_table_size(void* h, uint32_t table) {                                  \
         if (!h || table >= static_cast<Srv_##NAME*>(h)->num_tables()) return 0;                                   \

@@ -17,6 +17,13 @@
 // steps, so a large request does not hold up the small ones that arrive
 // behind it for longer than one bounded step.
 //
+// Latency mode: while the backlog is short (fewer than direct_backlog keys
+// pending), a request of at least direct_keys keys is a step of its own and
+// runs on its table's bulk kernels, the kernels of one plain call, which
+// keep more wavefronts per SIMD than the fused step kernel; batching only
+// pays for the small requests, and a long backlog (throughput) goes back
+// to full steps.
+//
 // Admission against the HBM budget: every table has a slot budget of
 // max_load * capacity. An insert of n keys reserves n slots of its table
 // (a key may be a duplicate, so this is an upper bound); slots are never
@@ -181,6 +188,7 @@ namespace serving {
 
 struct Stats {
     index_t requests = 0, steps = 0, keys = 0;
+    index_t direct_steps = 0;   // steps run on a table's own bulk kernels
 };
 
 // K/V: u32 or u64; CG: probing group (8 for 16-byte slots, 16 for 8-byte).
@@ -200,6 +208,11 @@ public:
         float max_load = 0.9f;                       // admission: slot budget per table
         int blocks_per_cu = 8;                       // persistent grid
         bool use_graph = true;                       // replay the step as a HIP graph
+        // Latency mode: while fewer than direct_backlog keys are pending, a
+        // request of at least direct_keys keys runs as a step of its own on
+        // its table's bulk kernels (0: every step runs the step kernel).
+        index_t direct_keys = index_t(1) << 18;
+        index_t direct_backlog = index_t(1) << 24;
     };
 
     struct Request {
@@ -364,18 +377,27 @@ private:
     // Requests (chunks) of the next step, chosen under mu_: per table in
     // arrival order, a run of lookups or one mutation; an insert is admitted
     // against its table's budget when its first chunk is taken. Writes the
-    // chunk descriptors to `descs`.
-    void pick(StepList& step, desc_type* descs) {
+    // chunk descriptors to `descs`. In latency mode (short backlog) a large
+    // request is taken alone, whole, and `direct` is set: the step runs on
+    // the table's own bulk kernels.
+    void pick(StepList& step, desc_type* descs, bool& direct) {
         std::vector<int> mode(tables_.size(), -1);   // -1 untouched, 0 lookups, 1 mutation, 2 closed
         std::deque<std::shared_ptr<Request>> keep;
         index_t keys = 0;
+        direct = false;
+        index_t backlog = 0;
+        if (cfg_.direct_keys != 0)
+            for (const auto& r : pending_) backlog += r->d.n - r->issued;
+        const bool latency = cfg_.direct_keys != 0 && backlog < cfg_.direct_backlog;
         while (!pending_.empty()) {
             auto r = pending_.front();
             pending_.pop_front();
             const std::uint32_t t = r->d.table;
             const Op op = Op(r->d.op);
-            bool take = step.size() < cfg_.max_requests && keys < cfg_.max_step_keys && mode[t] != 2 &&
+            bool take = !direct && step.size() < cfg_.max_requests && keys < cfg_.max_step_keys && mode[t] != 2 &&
                         mode[t] != 1 && (op == Op::retrieve || mode[t] == -1);
+            const bool large = latency && r->d.n - r->issued >= cfg_.direct_keys;
+            if (take && large && !step.empty()) take = false;   // runs alone in a later step
             if (take && op == Op::insert && r->issued == 0) {
                 if (reserved_[t] + r->d.n > budget_[t]) {
                     finish_refused(*r);
@@ -390,7 +412,8 @@ private:
             }
             const index_t left = r->d.n - r->issued;
             const index_t room = cfg_.max_step_keys - keys;
-            const index_t c = left < room ? left : room;
+            const index_t c = large || left < room ? left : room;
+            direct = large;
             desc_type d = r->d;
             d.keys += r->issued;
             if (d.values) d.values += r->issued;
@@ -472,6 +495,28 @@ private:
         return ok;
     }
 
+    // A step of one request on its table's own bulk kernels (the kernels of
+    // one plain call), on the step stream.
+    void launch_direct(const desc_type& d) {
+        table_type& t = *tables_[d.table];
+        const index_t pl = defaults::probing_length();
+        auto* st = reinterpret_cast<Status*>(d.status_out);
+        switch (Op(d.op)) {
+        case Op::insert:
+            if (st) t.template insert<status_handlers::ReturnStatus>(d.keys, d.values, d.n, stream_, pl, st);
+            else t.insert(d.keys, d.values, d.n, stream_, pl);
+            break;
+        case Op::retrieve:
+            if (st) t.template retrieve<status_handlers::ReturnStatus>(d.keys, d.n, d.values_out, stream_, pl, st);
+            else t.retrieve(d.keys, d.n, d.values_out, stream_, pl);
+            break;
+        case Op::erase:
+            if (st) t.template erase<status_handlers::ReturnStatus>(d.keys, d.n, stream_, pl, st);
+            else t.erase(d.keys, d.n, stream_, pl);
+            break;
+        }
+    }
+
     void launch(int b) {
         // the step reads its request count and descriptors from the pinned
         // staging buffer b, written by this thread before the launch
@@ -532,20 +577,26 @@ private:
         std::uint64_t step_no = 0;
         while (true) {
             StepList step;
+            bool direct = false;
             {
                 std::unique_lock<std::mutex> g(mu_);
                 if (inflight.empty()) cv_.wait(g, [&] { return stop_ || !pending_.empty(); });
                 if (stop_ && pending_.empty() && inflight.empty()) break;
                 // at most two steps in flight: descriptors are double-buffered
-                if (inflight.size() < 2) pick(step, desc_h_[b]);
+                if (inflight.size() < 2) pick(step, desc_h_[b], direct);
                 for (auto& c : step) c.r->state.store(1);
                 if (!step.empty()) stats_.steps++;
+                if (direct) stats_.direct_steps++;
             }
             if (!step.empty()) {
                 ++step_no;
-                num_reqs_h_[b][0] = std::uint32_t(step.size());
-                num_reqs_h_[b][1] = std::uint32_t(step_item_keys(step));
-                launch(b);
+                if (direct) {
+                    launch_direct(desc_h_[b][0]);
+                } else {
+                    num_reqs_h_[b][0] = std::uint32_t(step.size());
+                    num_reqs_h_[b][1] = std::uint32_t(step_item_keys(step));
+                    launch(b);
+                }
                 hipEvent_t ev = nullptr;
                 (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
                 (void)hipEventRecord(ev, stream_);

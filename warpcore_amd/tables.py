@@ -684,14 +684,18 @@ class Server:
 
     def __init__(self, table_capacities, key_bits: int = 64, value_bits: int = 64, max_requests: int = 0,
                  item_keys: int = 0, max_load: float = 0.9, use_graph: bool = True, blocks_per_cu: int = 0,
-                 max_step_keys: int = 0):
+                 max_step_keys: int = 0, direct_keys: int = 0):
+        """direct_keys: while the backlog is short, requests of at least this
+        many keys run alone on their table's bulk kernels (0: 2^18 keys,
+        None: never)."""
         import numpy as np
         self.key_bits, self.value_bits = key_bits, value_bits
         self._p = f"wc_server_{self._NAMES[(key_bits, value_bits)]}_"
         self._lib = _capi.lib()
         caps = np.ascontiguousarray(table_capacities, dtype=np.uint64)
         self._h = self._fn("create")(caps.ctypes.data_as(ctypes.c_void_p), len(caps), max_requests, item_keys,
-                                     max_step_keys, max_load, int(use_graph), blocks_per_cu)
+                                     max_step_keys, max_load, int(use_graph), blocks_per_cu,
+                                     (1 << 64) - 1 if direct_keys is None else direct_keys)
         if not self._h:
             raise ValueError("server could not be created")
 
@@ -725,7 +729,8 @@ class Server:
     def stats(self):
         out = (ctypes.c_uint64 * 3)()
         _check(self._fn("stats")(self._h, out), "stats")
-        return {"requests": out[0], "steps": out[1], "keys": out[2]}
+        return {"requests": out[0], "steps": out[1], "keys": out[2],
+                "direct_steps": self._fn("direct_steps")(self._h)}
 
     def table_size(self, table: int) -> int:
         return self._fn("table_size")(self._h, table)
