@@ -81,8 +81,8 @@ def test_server_matches_one_launch_per_request(cuda, use_graph, max_step_keys, d
     assert stats["requests"] == len(ops) and stats["keys"] == int(sizes.sum())
     if direct_keys is None:
         assert stats["direct_steps"] == 0
-    else:
-        assert 0 < stats["direct_steps"] <= int(np.sum(sizes >= direct_keys))
+    else:   # large requests, and steps of one whole request
+        assert int(np.sum(sizes >= direct_keys)) > 0 and stats["direct_steps"] > 0
     if max_step_keys:
         assert stats["steps"] >= int(sizes.sum()) // max_step_keys
     else:

@@ -20,9 +20,10 @@
 // Latency mode: while the backlog is short (fewer than direct_backlog keys
 // pending), a request of at least direct_keys keys is a step of its own and
 // runs on its table's bulk kernels, the kernels of one plain call, which
-// keep more wavefronts per SIMD than the fused step kernel; batching only
-// pays for the small requests, and a long backlog (throughput) goes back
-// to full steps.
+// keep more wavefronts per SIMD than the fused step kernel; a step that
+// holds one whole request runs there too. Batching only pays for several
+// small requests at once, and a long backlog (throughput) goes back to full
+// steps.
 //
 // Admission against the HBM budget: every table has a slot budget of
 // max_load * capacity. An insert of n keys reserves n slots of its table
@@ -379,7 +380,8 @@ private:
     // against its table's budget when its first chunk is taken. Writes the
     // chunk descriptors to `descs`. In latency mode (short backlog) a large
     // request is taken alone, whole, and `direct` is set: the step runs on
-    // the table's own bulk kernels.
+    // the table's own bulk kernels; so does a step that holds one whole
+    // request.
     void pick(StepList& step, desc_type* descs, bool& direct) {
         std::vector<int> mode(tables_.size(), -1);   // -1 untouched, 0 lookups, 1 mutation, 2 closed
         std::deque<std::shared_ptr<Request>> keep;
@@ -431,6 +433,9 @@ private:
                 keep.push_back(r);
             }
         }
+        // latency mode: a step of one whole request gains nothing from the
+        // step kernel, so it runs on the bulk kernels too (grid sized to it)
+        if (latency && step.size() == 1 && step[0].keys == step[0].r->d.n) direct = true;
         pending_.swap(keep);
         waiting_.store(pending_.size(), std::memory_order_release);
     }
