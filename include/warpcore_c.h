@@ -616,8 +616,10 @@ WC_API uint64_t wc_get_lane_insert_table_bytes(void);
  * counterpart), a bit mask: bit 3 = refill inserts, bit 2 = refill lookups,
  * bit 0 = lockstep inserts, bit 1 = lockstep lookups and erases (a refill
  * bit wins over the lockstep one; no bit = the streaming per-key state
- * machine). Default 12 (refill inserts and lookups). All choices give
- * identical results. */
+ * machine). Default 12 (refill inserts and lookups). MultiValueHashTable
+ * bulk calls use the refill kernels under bits 3 (inserts) and 2 (count and
+ * gather passes), else one key per group. All choices give identical
+ * results. */
 WC_API void wc_set_group_kernel(int kernel);
 WC_API int wc_get_group_kernel(void);
 
