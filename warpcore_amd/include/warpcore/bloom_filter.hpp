@@ -107,19 +107,21 @@ __global__ __launch_bounds__(256) void bloom_insert(const typename View::key_typ
     const index_t groups = (index_t(gridDim.x) * blockDim.x) / CG;
     for (index_t base = gid; base < num_in; base += groups * U) {
         index_t at[U];
-        S pat[U], cur[U];
+        S pat[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const index_t i = base + index_t(u) * groups;
             const K key = keys_in[i < num_in ? i : num_in - 1];
             view.locate(key, rank, at[u], pat[u]);
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u) cur[u] = view.words[at[u]];
+        // OR without reading the word first: a non-returning atomic that
+        // nothing waits for (the per-key device insert reads first and skips
+        // words that already hold the pattern; the bit array is the same)
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const index_t i = base + index_t(u) * groups;
-            if (i < num_in && (pat[u] & cur[u]) != pat[u]) View::atomic_or(view.words + at[u], pat[u]);
+            if (i < num_in && pat[u] != 0)   // a lane's word of the block may get no bit
+                (void)__hip_atomic_fetch_or(view.words + at[u], pat[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
