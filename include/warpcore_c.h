@@ -57,10 +57,14 @@
  *   _init / _init_seed init(stream) / init(seed, stream)           :175-201
  *   _insert            insert<ReturnStatus or ReturnNothing>(...)  :239-258
  *   _insert_bool       insert<ReturnBoolean<>>(...)                :239-258
+ *                      (flag = has_any(~none()): invalid_value only,
+ *                      status.cuh:146-150,316-333)
  *   _retrieve          retrieve<ReturnStatus or ReturnNothing>     :318-337
  *   _retrieve_bool     retrieve<ReturnBoolean<>>                   :318-337
  *   _erase             erase(...)                                  :478-496
  *   _retrieve_all      retrieve_all(keys_out, values_out, num_out) :390-417
+ *   _for_each_count    for_each(f) with f adding one to *counter_d per
+ *                      stored pair (device counter, caller-zeroed) :504-528
  *   _size _load_factor _capacity _bytes_total                     :534-630
  *   _peek_status _pop_status _is_initialized                       :635-687
  *   _cg_size           cg_size()                                   :100-104
@@ -77,6 +81,7 @@ WC_API int wc_svht_u32u32_retrieve(void* table, const uint32_t* keys_in, uint64_
 WC_API int wc_svht_u32u32_retrieve_bool(void* table, const uint32_t* keys_in, uint64_t num_in, uint32_t* values_out, void* stream, uint64_t probing_length, bool* flags_out);
 WC_API int wc_svht_u32u32_erase(void* table, const uint32_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_svht_u32u32_retrieve_all(void* table, uint32_t* keys_out, uint32_t* values_out, uint64_t* num_out, void* stream);
+WC_API int wc_svht_u32u32_for_each_count(void* table, uint64_t* counter_d, void* stream);
 WC_API uint64_t wc_svht_u32u32_size(void* table, void* stream);
 WC_API float wc_svht_u32u32_load_factor(void* table, void* stream);
 WC_API uint64_t wc_svht_u32u32_capacity(void* table);
@@ -97,6 +102,7 @@ WC_API int wc_svht_u64u64_retrieve(void* table, const uint64_t* keys_in, uint64_
 WC_API int wc_svht_u64u64_retrieve_bool(void* table, const uint64_t* keys_in, uint64_t num_in, uint64_t* values_out, void* stream, uint64_t probing_length, bool* flags_out);
 WC_API int wc_svht_u64u64_erase(void* table, const uint64_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_svht_u64u64_retrieve_all(void* table, uint64_t* keys_out, uint64_t* values_out, uint64_t* num_out, void* stream);
+WC_API int wc_svht_u64u64_for_each_count(void* table, uint64_t* counter_d, void* stream);
 WC_API uint64_t wc_svht_u64u64_size(void* table, void* stream);
 WC_API float wc_svht_u64u64_load_factor(void* table, void* stream);
 WC_API uint64_t wc_svht_u64u64_capacity(void* table);
@@ -117,6 +123,7 @@ WC_API int wc_svht_u32u64_retrieve(void* table, const uint32_t* keys_in, uint64_
 WC_API int wc_svht_u32u64_retrieve_bool(void* table, const uint32_t* keys_in, uint64_t num_in, uint64_t* values_out, void* stream, uint64_t probing_length, bool* flags_out);
 WC_API int wc_svht_u32u64_erase(void* table, const uint32_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_svht_u32u64_retrieve_all(void* table, uint32_t* keys_out, uint64_t* values_out, uint64_t* num_out, void* stream);
+WC_API int wc_svht_u32u64_for_each_count(void* table, uint64_t* counter_d, void* stream);
 WC_API uint64_t wc_svht_u32u64_size(void* table, void* stream);
 WC_API float wc_svht_u32u64_load_factor(void* table, void* stream);
 WC_API uint64_t wc_svht_u32u64_capacity(void* table);
@@ -137,6 +144,7 @@ WC_API int wc_svht_u64u32_retrieve(void* table, const uint64_t* keys_in, uint64_
 WC_API int wc_svht_u64u32_retrieve_bool(void* table, const uint64_t* keys_in, uint64_t num_in, uint32_t* values_out, void* stream, uint64_t probing_length, bool* flags_out);
 WC_API int wc_svht_u64u32_erase(void* table, const uint64_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_svht_u64u32_retrieve_all(void* table, uint64_t* keys_out, uint32_t* values_out, uint64_t* num_out, void* stream);
+WC_API int wc_svht_u64u32_for_each_count(void* table, uint64_t* counter_d, void* stream);
 WC_API uint64_t wc_svht_u64u32_size(void* table, void* stream);
 WC_API float wc_svht_u64u32_load_factor(void* table, void* stream);
 WC_API uint64_t wc_svht_u64u32_capacity(void* table);
@@ -146,6 +154,60 @@ WC_API uint32_t wc_svht_u64u32_pop_status(void* table, void* stream);
 WC_API int wc_svht_u64u32_is_initialized(void* table);
 WC_API uint32_t wc_svht_u64u32_cg_size(void* table);
 
+
+/* ------------------------------------------------------------------------
+ * SingleValueHashTable with another ProbingScheme, Hasher or TableStorage
+ * template argument (include/warpcore/probing_schemes.cuh:22-302,
+ * hashers.cuh:14-175, storage.cuh:218-645). The handle is that of
+ * wc_svht_*_create and takes every wc_svht_* entry point of its (key,
+ * value) pair. Returns NULL for a combination that is not compiled; the
+ * compiled ones, all with cg_size 8 (0 = 8) and hasher1 == hasher2:
+ *   probing DOUBLE    hasher MURMUR, MUELLER*, NVIDIA*, IDENTITY   storage AOS
+ *   probing LINEAR    hasher MURMUR, IDENTITY                      storage AOS
+ *   probing QUADRATIC hasher MURMUR, IDENTITY                      storage AOS
+ *   probing DOUBLE    hasher MURMUR                                storage SOA
+ * (* 32-bit keys only, as in the reference). Windows are aligned to cg
+ * slots as for the default scheme; LINEAR steps one window, QUADRATIC
+ * steps 1, 2, 3, ... windows.
+ * ---------------------------------------------------------------------- */
+#define WC_PROBING_DOUBLE 0
+#define WC_PROBING_LINEAR 1
+#define WC_PROBING_QUADRATIC 2
+#define WC_HASHER_MURMUR 0
+#define WC_HASHER_MUELLER 1
+#define WC_HASHER_NVIDIA 2
+#define WC_HASHER_IDENTITY 3
+#define WC_STORAGE_AOS 0
+#define WC_STORAGE_SOA 1
+WC_API void* wc_svht_u32u32_create_custom(uint64_t min_capacity, uint32_t seed, int no_init, uint32_t cg_size, int probing, int hasher1, int hasher2, int storage);
+WC_API void* wc_svht_u64u64_create_custom(uint64_t min_capacity, uint64_t seed, int no_init, uint32_t cg_size, int probing, int hasher1, int hasher2, int storage);
+
+/* ------------------------------------------------------------------------
+ * Status algebra evaluated on the host (include/warpcore/status.cuh:16-333),
+ * bit for bit the reference's: all() = 0xFFF (invalid_value is outside it),
+ * ~s = (~s) ^ 0xFFF, a - b = a & ~b, a + b = a | b.
+ *   _return_boolean    the flag status_handlers::ReturnBoolean<ignore>
+ *                      writes: has_any(~Status(ignore)) (:316-333); with
+ *                      ignore = 0 it is set only by invalid_value.
+ *   _return_any_flag   status_handlers::ReturnAnyFlag<ignore> (no reference
+ *                      counterpart): any of the 13 flags outside ignore.
+ *   _to_string         operator<< text (:216-269); returns its length.
+ * ---------------------------------------------------------------------- */
+WC_API uint32_t wc_status_all(void);
+WC_API uint32_t wc_status_error_mask(void);
+WC_API uint32_t wc_status_warning_mask(void);
+WC_API uint32_t wc_status_complement(uint32_t status);
+WC_API uint32_t wc_status_join(uint32_t a, uint32_t b);
+WC_API uint32_t wc_status_remove(uint32_t a, uint32_t b);
+WC_API int wc_status_has_any(uint32_t status, uint32_t flags);
+WC_API int wc_status_has_all(uint32_t status, uint32_t flags);
+WC_API int wc_status_has_any_errors(uint32_t status);
+WC_API int wc_status_has_any_warnings(uint32_t status);
+WC_API uint32_t wc_status_get_errors(uint32_t status);
+WC_API uint32_t wc_status_get_warnings(uint32_t status);
+WC_API int wc_status_return_boolean(uint32_t status, uint32_t ignore);
+WC_API int wc_status_return_any_flag(uint32_t status, uint32_t ignore);
+WC_API int wc_status_to_string(uint32_t status, char* buf, uint64_t len);
 
 /* ------------------------------------------------------------------------
  * CountingHashTable<K, uint64_t> (include/warpcore/counting_hash_table.cuh)
@@ -200,6 +262,7 @@ WC_API uint32_t wc_counting_u64_cg_size(void* table);
  *             the device retrieve semantics of :291-329 (the host retrieve
  *             of :340-359 cannot be instantiated)
  *   _erase (status none whatever it finds)                         :446-462
+ *   _for_each_count for_each(f), f adding one to *counter_d per key  :472
  *   _retrieve_all / _size / _load_factor / _capacity / _peek_status /
  *   _pop_status                                                    :366-640
  * cg_size 1..64, default 16.
@@ -213,6 +276,7 @@ WC_API int wc_hashset_u32_insert(void* set, const uint32_t* keys_in, uint64_t nu
 WC_API int wc_hashset_u32_contains(void* set, const uint32_t* keys_in, uint64_t num_in, bool* flags_out, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_hashset_u32_erase(void* set, const uint32_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_hashset_u32_retrieve_all(void* set, uint32_t* keys_out, uint64_t* num_out, void* stream);
+WC_API int wc_hashset_u32_for_each_count(void* set, uint64_t* counter_d, void* stream);
 WC_API uint64_t wc_hashset_u32_size(void* set, void* stream);
 WC_API float wc_hashset_u32_load_factor(void* set, void* stream);
 WC_API uint64_t wc_hashset_u32_capacity(void* set);
@@ -230,6 +294,7 @@ WC_API int wc_hashset_u64_insert(void* set, const uint64_t* keys_in, uint64_t nu
 WC_API int wc_hashset_u64_contains(void* set, const uint64_t* keys_in, uint64_t num_in, bool* flags_out, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_hashset_u64_erase(void* set, const uint64_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_hashset_u64_retrieve_all(void* set, uint64_t* keys_out, uint64_t* num_out, void* stream);
+WC_API int wc_hashset_u64_for_each_count(void* set, uint64_t* counter_d, void* stream);
 WC_API uint64_t wc_hashset_u64_size(void* set, void* stream);
 WC_API float wc_hashset_u64_load_factor(void* set, void* stream);
 WC_API uint64_t wc_hashset_u64_capacity(void* set);

@@ -28,6 +28,12 @@ LIB_PATH = HERE / "liboracle.so"
 
 # probing schemes
 REF_DOUBLE, REF_LINEAR, REF_QUADRATIC, AL_DOUBLE, AL_LINEAR, AL_QUADRATIC = range(6)
+# hashers of a probing scheme: pass `scheme | (hasher << 4)` wherever a scheme is taken
+H_MURMUR, H_MUELLER, H_NVIDIA, H_IDENTITY = range(4)
+
+
+def with_hasher(scheme: int, hasher: int) -> int:
+    return scheme | (hasher << 4)
 
 S_NONE = 0
 S_UNKNOWN = 1 << 0
@@ -52,6 +58,9 @@ _u32, _u64, _i, _vp, _f, _d = (ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int, c
 _SIGS = {
     "wco_fmix32": (_u32, [_u32]), "wco_fmix64": (_u64, [_u64]),
     "wco_nvidia_hash": (_u32, [_u32]), "wco_mueller_hash": (_u32, [_u32]),
+    "wco_status_all": (_u32, []), "wco_status_error_mask": (_u32, []), "wco_status_warning_mask": (_u32, []),
+    "wco_status_not": (_u32, [_u32]), "wco_status_remove": (_u32, [_u32, _u32]),
+    "wco_status_return_boolean": (_i, [_u32, _u32]),
     "wco_fmix32_array": (None, [_vp, _vp, _u64]), "wco_fmix64_array": (None, [_vp, _vp, _u64]),
     "wco_num_primes": (_u64, []), "wco_primes": (None, [_vp, _u64]), "wco_is_prime": (_i, [_u64]),
     "wco_get_valid_capacity": (_u64, [_u64, _u64]),
@@ -128,6 +137,32 @@ def fmix32(x: int) -> int:
 
 def fmix64(x: int) -> int:
     return lib().wco_fmix64(x)
+
+
+def status_all() -> int:
+    return lib().wco_status_all()
+
+
+def status_error_mask() -> int:
+    return lib().wco_status_error_mask()
+
+
+def status_warning_mask() -> int:
+    return lib().wco_status_warning_mask()
+
+
+def status_not(s: int) -> int:
+    return lib().wco_status_not(s)
+
+
+def status_remove(a: int, b: int) -> int:
+    return lib().wco_status_remove(a, b)
+
+
+def return_boolean(statuses, ignore: int = 0) -> np.ndarray:
+    """Flags status_handlers::ReturnBoolean<ignore> writes for `statuses`."""
+    mask = np.uint32(status_not(ignore))
+    return ((np.asarray(statuses, np.uint32) & mask) != 0).astype(np.uint8)
 
 
 def get_valid_capacity(min_capacity: int, cg: int) -> int:

@@ -96,6 +96,14 @@ u32 wco_fmix32(u32 x) { return fmix32(x); }
 u64 wco_fmix64(u64 x) { return fmix64(x); }
 u32 wco_nvidia_hash(u32 x) { return nvidia_hash(x); }
 u32 wco_mueller_hash(u32 x) { return mueller_hash(x); }
+
+// ----------------------------------------------------------------- status --
+u32 wco_status_all() { return status_all(); }
+u32 wco_status_error_mask() { return status_error_mask(); }
+u32 wco_status_warning_mask() { return status_warning_mask(); }
+u32 wco_status_not(u32 s) { return status_not(s); }
+u32 wco_status_remove(u32 a, u32 b) { return status_remove(a, b); }
+int wco_status_return_boolean(u32 s, u32 ignore) { return status_return_boolean(s, ignore) ? 1 : 0; }
 void wco_fmix32_array(const u32* in, u32* out, u64 n) { for (u64 i = 0; i < n; ++i) out[i] = fmix32(in[i]); }
 void wco_fmix64_array(const u64* in, u64* out, u64 n) { for (u64 i = 0; i < n; ++i) out[i] = fmix64(in[i]); }
 

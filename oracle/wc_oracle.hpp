@@ -56,6 +56,20 @@ template<class K> inline K murmur(K x);
 template<> inline u32 murmur<u32>(u32 x) { return fmix32(x); }
 template<> inline u64 murmur<u64>(u64 x) { return fmix64(x); }
 
+// Hasher template argument of a probing scheme: MurmurHash (hashers.cuh:85-141),
+// MuellerHash (:50-79), NvidiaHash (:14-46), IdentityMap (:147-175). The
+// Mueller and Nvidia hashers exist for 32-bit keys only.
+enum Hasher : int { H_MURMUR = 0, H_MUELLER = 1, H_NVIDIA = 2, H_IDENTITY = 3 };
+template<class K>
+inline K hash_with(int hasher, K x) {
+    switch (hasher) {
+    case H_MUELLER: return K(mueller_hash(u32(x)));
+    case H_NVIDIA: return K(nvidia_hash(u32(x)));
+    case H_IDENTITY: return x;
+    default: return murmur<K>(x);
+    }
+}
+
 // ----------------------------------------------------------------- status --
 // bit positions: include/warpcore/status.cuh:44-68
 enum : u32 {
@@ -74,6 +88,21 @@ enum : u32 {
     S_MAXV = 1u << 11,           // max_values_for_key_reached
     S_INVALID_VALUE = 1u << 12,
 };
+
+// Status algebra, include/warpcore/status.cuh: mask 0xFFF (:207), so
+// all() = 0xFFF (:40) and invalid_value is outside it; error_mask (:71-80);
+// warning_mask = all() - error_mask() with a - b = a - (a & b) (:82-86,
+// :160-170); ~s = (~s) ^ mask (:146-150); the flag ReturnBoolean<Ignore>
+// writes is has_any(~Status(Ignore)) (:316-333).
+constexpr u32 STATUS_MASK = 0xFFFu;
+inline u32 status_all() { return STATUS_MASK; }
+inline u32 status_error_mask() {
+    return S_UNKNOWN | S_INVALID_CONFIG | S_INDEX_OVERFLOW | S_OOM | S_NOT_INIT | S_PHASE;
+}
+inline u32 status_remove(u32 a, u32 b) { return a - (a & b); }
+inline u32 status_warning_mask() { return status_remove(status_all(), status_error_mask()); }
+inline u32 status_not(u32 s) { return (~s) ^ STATUS_MASK; }
+inline bool status_return_boolean(u32 s, u32 ignore) { return (s & status_not(ignore)) != 0; }
 
 // ----------------------------------------------------------------- primes --
 // Capacity rule include/warpcore/base.cuh:27-34 over the prime table whose
@@ -97,14 +126,15 @@ enum Scheme : int {
 // Iterates the windows of one key. slots[r] is lane r's slot in the current
 // window. Window count: SDIV(pl, cg) (probing_schemes.cuh:61), capped at one
 // full cycle capacity/cg.
+// `scheme_` carries the Hasher in bits 4..7 (0 = MurmurHash).
 template<class K>
 struct Windows {
-    int scheme; u64 cap, cg, p, left;
+    int scheme, hasher; u64 cap, cg, p, left;
     u64 pos[64];
     u64 step;  // double: step in slots (ref) or windows (aligned); quad: counter
     u64 w;     // aligned window index
 
-    Windows(int scheme_, u64 cap_, u64 cg_, u64 pl) : scheme(scheme_), cap(cap_), cg(cg_) {
+    Windows(int scheme_, u64 cap_, u64 cg_, u64 pl) : scheme(scheme_ & 15), hasher(scheme_ >> 4), cap(cap_), cg(cg_) {
         p = cap / cg;
         // The first window is always probed (begin() is not checked against
         // the probing length, single_value_hash_table.cuh:764); next() stops
@@ -114,8 +144,9 @@ struct Windows {
         left = std::min(windows, p);
     }
     bool begin(K key, K seed) {
-        const K h1 = murmur<K>(K(key + seed));
-        const K h2 = murmur<K>(K(key + seed + K(1)));
+        // probing_schemes.cuh:79-81 (double), :176 (linear), :267 (quadratic)
+        const K h1 = hash_with<K>(hasher, K(key + seed));
+        const K h2 = hash_with<K>(hasher, K(key + seed + K(1)));
         switch (scheme) {
         case REF_DOUBLE:
         case REF_LINEAR:

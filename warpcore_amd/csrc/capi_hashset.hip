@@ -9,6 +9,13 @@ using wc_capi::last_error;
 
 namespace {
 
+// for_each functor of count_valid: one device counter increment per key
+struct CountKeys {
+    unsigned long long* counter;
+    template<class K>
+    __device__ void operator()(const K&) const noexcept { atomicAdd(counter, 1ull); }
+};
+
 template<class K>
 struct SetApi {
     virtual ~SetApi() = default;
@@ -17,6 +24,7 @@ struct SetApi {
     virtual void contains(const K*, index_t, bool*, hipStream_t, index_t, Status*) = 0;
     virtual void erase(const K*, index_t, hipStream_t, index_t, Status*) = 0;
     virtual void retrieve_all(K*, index_t&, hipStream_t) = 0;
+    virtual void count_valid(index_t*, hipStream_t) = 0;
     virtual index_t size(hipStream_t) = 0;
     virtual float load_factor(hipStream_t) = 0;
     virtual index_t capacity() = 0;
@@ -45,6 +53,9 @@ struct SetImpl final : SetApi<K> {
         else t.erase(k, n, s, pl);
     }
     void retrieve_all(K* k, index_t& n, hipStream_t s) override { t.retrieve_all(k, n, s); }
+    void count_valid(index_t* counter, hipStream_t s) override {
+        t.for_each(CountKeys{reinterpret_cast<unsigned long long*>(counter)}, s);
+    }
     index_t size(hipStream_t s) override { return t.size(s); }
     float load_factor(hipStream_t s) override { return t.load_factor(s); }
     index_t capacity() override { return t.capacity(); }
@@ -105,6 +116,11 @@ SetApi<K>* make(index_t min_cap, K seed, unsigned cg) {
         static_cast<SetApi<K>*>(h)->erase(keys, n, as_stream(stream), pl, reinterpret_cast<Status*>(status_out)); \
         return last_error();                                                                                      \
     }                                                                                                             \
+    extern "C" int wc_hashset_##NAME##_for_each_count(void* h, uint64_t* counter_d, void* stream) {              \
+        if (!h || !counter_d) return wc_capi::kBadHandle;                                                          \
+        static_cast<SetApi<K>*>(h)->count_valid(counter_d, as_stream(stream));                                    \
+        return last_error();                                                                                       \
+    }                                                                                                              \
     extern "C" int wc_hashset_##NAME##_retrieve_all(void* h, K* keys_out, uint64_t* num_out, void* stream) {      \
         if (!h || !num_out) return wc_capi::kBadHandle;                                                           \
         index_t n = 0;                                                                                            \

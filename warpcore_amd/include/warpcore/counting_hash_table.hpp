@@ -231,7 +231,7 @@ __global__ __launch_bounds__(256) void count_join_duplicates(const V* __restrict
     const index_t stride = index_t(gridDim.x) * blockDim.x;
     for (index_t e = index_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += stride) {
         const Status s = entry_status[e];
-        if (counts[e] > 1 && !s.has_any(~(Status::duplicate_key() + Status::index_overflow())))
+        if (counts[e] > 1 && !s.has_any(Status::flags_except(Status::duplicate_key() + Status::index_overflow())))
             join(Status::duplicate_key());
     }
     join.flush(word);
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void count_expand(const std::uint32_t* __restr
         Status s = entry_status[e];
         if (rep[e] != std::uint32_t(i)) {
             const Status stored = s - Status::index_overflow();
-            if (!stored.has_any(~Status::duplicate_key())) s = Status::duplicate_key() + (s - stored);
+            if (!stored.has_any(Status::flags_except(Status::duplicate_key()))) s = Status::duplicate_key() + (s - stored);
         }
         StatusHandler::handle(s, status_out, first + i);
     }

@@ -265,6 +265,143 @@ struct SingleValueView {
     }
 };
 
+// Device state of a single-value table over storage::key_value::SoAStore:
+// keys and values in two arrays. The per-window protocol is the one of
+// SingleValueView (single_value_hash_table.cuh:742-823, 267-307, 425-468);
+// a lane reads only its slot's key, the CAS publishes the key and the value
+// is stored after it, and a lookup reads the value of the hit slot alone.
+// Bulk calls run the one-group-per-key kernels of engine.hpp.
+template<class Key, class Value, Key EmptyKey, Key TombstoneKey, class ProbingScheme>
+struct SingleValueSoAView {
+    using key_type = Key;
+    using value_type = Value;
+    using slot_type = storage::key_value::pair_t<Key, Value>;
+    using index_type = index_t;
+    using probing_scheme_type = ProbingScheme;
+    using out_type = Value;
+
+    Key* keys;
+    Value* values;
+    Status* status;
+    probing_schemes::Geometry geo;
+    Key seed;
+    bool initialized;
+
+    WC_HD static constexpr index_type cg_size() noexcept { return ProbingScheme::cg_size(); }
+    WC_HD static constexpr bool is_empty_key(Key k) noexcept { return k == EmptyKey; }
+    WC_HD static constexpr bool is_tombstone_key(Key k) noexcept { return k == TombstoneKey; }
+    WC_HD static constexpr bool is_valid_key(Key k) noexcept { return k != EmptyKey && k != TombstoneKey; }
+    WC_HD index_type capacity() const noexcept { return geo.capacity; }
+    WC_HD Status* status_word() const noexcept { return status; }
+    WC_D Key key_at(index_type i) const noexcept { return keys[i]; }
+    WC_D slot_type slot_at(index_type i) const noexcept {
+        slot_type s;
+        s.key = keys[i];
+        s.value = values[i];
+        return s;
+    }
+
+    // Slot of `key_in` (found or claimed), ~0 if none.
+    template<class Join>
+    WC_D index_type insert_impl(Key key_in, Status& status_out, const tile<cg_size()>& group,
+                                index_type probing_length, Join& join) const noexcept {
+        if (!initialized) { status_out = Status::not_initialized(); return ~index_type(0); }
+        if (!is_valid_key(key_in)) {
+            join(Status::invalid_key());
+            status_out = Status::invalid_key();
+            return ~index_type(0);
+        }
+        ProbingScheme iter(geo, probing_length, group);
+        for (index_type i = iter.begin(key_in, seed); i != iter.end(); i = iter.next()) {
+            const Key table_key = keys[i];
+            const auto hit_mask = group.ballot(table_key == key_in);
+            if (hit_mask) {
+                join(Status::duplicate_key());
+                status_out = Status::duplicate_key();
+                return group.shfl(i, unsigned(ffs64(hit_mask) - 1));
+            }
+            auto empty_mask = group.ballot(is_empty_key(table_key));
+            while (empty_mask) {
+                const unsigned leader = unsigned(ffs64(empty_mask) - 1);
+                bool success = false;
+                bool duplicate = false;
+                if (group.thread_rank() == leader) {
+                    const Key old = atomic_cas(keys + i, table_key, key_in);
+                    success = old == table_key;
+                    duplicate = old == key_in;
+                }
+                if (group.any(duplicate)) {
+                    join(Status::duplicate_key());
+                    status_out = Status::duplicate_key();
+                    return group.shfl(i, leader);
+                }
+                if (group.any(success)) {
+                    status_out = Status::none();
+                    return group.shfl(i, leader);
+                }
+                empty_mask ^= std::uint64_t(1) << leader;
+            }
+        }
+        join(Status::probing_length_exceeded());
+        status_out = Status::probing_length_exceeded();
+        return ~index_type(0);
+    }
+
+    template<class Join>
+    WC_D Status insert(Key key_in, const Value& value_in, const tile<cg_size()>& group, index_type probing_length,
+                       Join& join) const noexcept {
+        Status status = Status::unknown_error();
+        const index_type s = insert_impl(key_in, status, group, probing_length, join);
+        if (group.thread_rank() == 0 && s != ~index_type(0)) values[s] = value_in;
+        return status;
+    }
+
+    template<class Join>
+    WC_D Status retrieve(Key key_in, Value& value_out, const tile<cg_size()>& group, index_type probing_length,
+                         Join& join) const noexcept {
+        if (!initialized) return Status::not_initialized();
+        if (!is_valid_key(key_in)) { join(Status::invalid_key()); return Status::invalid_key(); }
+        ProbingScheme iter(geo, probing_length, group);
+        for (index_type i = iter.begin(key_in, seed); i != iter.end(); i = iter.next()) {
+            const Key table_key = keys[i];
+            const auto hit_mask = group.ballot(table_key == key_in);
+            if (hit_mask) {
+                const unsigned at = unsigned(ffs64(hit_mask) - 1);
+                Value v{};
+                if (group.thread_rank() == at) v = values[i];
+                value_out = group.shfl(v, at);
+                return Status::none();
+            }
+            if (group.any(is_empty_key(table_key))) {
+                join(Status::key_not_found());
+                return Status::key_not_found();
+            }
+        }
+        join(Status::probing_length_exceeded());
+        return Status::probing_length_exceeded();
+    }
+
+    template<class Join>
+    WC_D Status erase(Key key_in, const tile<cg_size()>& group, index_type probing_length, Join& join) const noexcept {
+        if (!initialized) return Status::not_initialized();
+        if (!is_valid_key(key_in)) { join(Status::invalid_key()); return Status::invalid_key(); }
+        ProbingScheme iter(geo, probing_length, group);
+        for (index_type i = iter.begin(key_in, seed); i != iter.end(); i = iter.next()) {
+            const Key table_key = keys[i];
+            const auto hit_mask = group.ballot(table_key == key_in);
+            if (hit_mask) {
+                if (group.thread_rank() == unsigned(ffs64(hit_mask) - 1)) keys[i] = TombstoneKey;
+                return Status::none();
+            }
+            if (group.any(is_empty_key(table_key))) return Status::key_not_found();
+        }
+        return Status::probing_length_exceeded();
+    }
+};
+
+template<class Storage> struct is_soa_store : std::false_type {};
+template<class K, class V> struct is_soa_store<storage::key_value::SoAStore<K, V>> : std::true_type {};
+
 }  // namespace detail
 
 template<class Key, class Value, Key EmptyKey = defaults::empty_key<Key>(),
@@ -294,7 +431,10 @@ public:
     using status_type = Status;
     using probing_scheme_type = ProbingScheme;
     using slot_type = typename TableStorage::slot_type;
-    using view_type = detail::SingleValueView<Key, Value, EmptyKey, TombstoneKey, ProbingScheme, slot_type>;
+    static constexpr bool soa_storage = detail::is_soa_store<TableStorage>::value;
+    using view_type = typename std::conditional<
+        soa_storage, detail::SingleValueSoAView<Key, Value, EmptyKey, TombstoneKey, ProbingScheme>,
+        detail::SingleValueView<Key, Value, EmptyKey, TombstoneKey, ProbingScheme, slot_type>>::type;
 
     WC_HD static constexpr key_type empty_key() noexcept { return EmptyKey; }
     WC_HD static constexpr key_type tombstone_key() noexcept { return TombstoneKey; }
@@ -368,30 +508,12 @@ public:
                      typename StatusHandler::base_type* status_out = nullptr) noexcept {
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!is_initialized_ || num_in == 0) return;
-        if constexpr (detail::LaneWindow<slot_type, cg_size()>::usable) {
-            if (capacity() * sizeof(slot_type) <= detail::lane_insert_table_bytes()) {
-                kernels::insert_lanes<view_type, StatusHandler, detail::keys_per_lane(), true>
-                    <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, view(),
-                                                                                 probing_length, status_out);
-                return;
-            }
-        }
-        if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
-            if (detail::refill_inserts()) {
-                detail::launch_insert_refill<view_type, StatusHandler, detail::refill_insert_keys<unsigned(cg_size())>(), true>(
-                    keys_in, values_in, num_in, view(), probing_length, status_out, stream);
-                return;
-            }
-            if (detail::lockstep_inserts()) {
-                detail::launch_insert_lockstep<view_type, StatusHandler,
-                                               detail::lockstep_insert_keys<unsigned(cg_size())>(), true>(
-                    keys_in, values_in, num_in, view(), probing_length, status_out, stream);
-                return;
-            }
-        }
-        kernels::insert_batched<view_type, StatusHandler, detail::keys_per_group(), true>
-            <<<detail::grid_for_batched(num_in, cg_size()), block_size(), 0, stream>>>(
+        if constexpr (soa_storage) {
+            kernels::insert<view_type, StatusHandler><<<detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(
                 keys_in, values_in, num_in, view(), probing_length, status_out);
+        } else {
+            insert_aos<StatusHandler>(keys_in, values_in, num_in, stream, probing_length, status_out);
+        }
     }
 
     // single_value_hash_table.cuh:318-337
@@ -401,29 +523,12 @@ public:
                        typename StatusHandler::base_type* status_out = nullptr) const noexcept {
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!is_initialized_ || num_in == 0) return;
-        if constexpr (detail::LaneWindow<slot_type, cg_size()>::usable) {
-            if (capacity() * sizeof(slot_type) <= detail::lane_table_bytes()) {
-                kernels::retrieve_lanes<view_type, StatusHandler, detail::keys_per_lane()>
-                    <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, num_in, values_out, view(),
-                                                                                 probing_length, status_out);
-                return;
-            }
-        }
-        if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
-            if (detail::refill_lookups()) {
-                detail::launch_retrieve_refill<view_type, StatusHandler, detail::refill_keys<unsigned(cg_size())>()>(
-                    keys_in, num_in, values_out, view(), probing_length, status_out, stream);
-                return;
-            }
-            if (detail::lockstep_lookups()) {
-                detail::launch_retrieve_lockstep<view_type, StatusHandler, detail::lockstep_keys<unsigned(cg_size())>()>(
-                    keys_in, num_in, values_out, view(), probing_length, status_out, stream);
-                return;
-            }
-        }
-        kernels::retrieve_batched<view_type, StatusHandler, detail::keys_per_group()>
-            <<<detail::grid_for_batched(num_in, cg_size()), block_size(), 0, stream>>>(
+        if constexpr (soa_storage) {
+            kernels::retrieve<view_type, StatusHandler><<<detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(
                 keys_in, num_in, values_out, view(), probing_length, status_out);
+        } else {
+            retrieve_aos<StatusHandler>(keys_in, num_in, values_out, stream, probing_length, status_out);
+        }
     }
 
     // single_value_hash_table.cuh:390-417 (order unspecified)
@@ -445,7 +550,7 @@ public:
                     typename StatusHandler::base_type* status_out = nullptr) noexcept {
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!is_initialized_ || num_in == 0) return;
-        if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+        if constexpr (!soa_storage && detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
             if (detail::lockstep_lookups()) {
                 detail::launch_erase_lockstep<view_type, StatusHandler, detail::lockstep_keys<unsigned(cg_size())>()>(
                     keys_in, num_in, view(), probing_length, status_out, stream);
@@ -513,10 +618,73 @@ public:
 
     // Device state handed to kernels.
     WC_HD view_type view() const noexcept {
-        return view_type{table_.data(), status_, geo_, seed_, is_initialized_};
+        if constexpr (soa_storage) {
+            return view_type{table_.keys(), table_.values(), status_, geo_, seed_, is_initialized_};
+        } else {
+            return view_type{table_.data(), status_, geo_, seed_, is_initialized_};
+        }
     }
 
 private:
+    // Bulk kernels of AoS tables: lane-per-key windows on small tables,
+    // refill (default), lockstep or streaming cooperative kernels above.
+    template<class StatusHandler>
+    WC_H void insert_aos(const key_type* keys_in, const value_type* values_in, index_type num_in, hipStream_t stream,
+                         index_type probing_length, typename StatusHandler::base_type* status_out) noexcept {
+        if constexpr (detail::LaneWindow<slot_type, cg_size()>::usable) {
+            if (capacity() * sizeof(slot_type) <= detail::lane_insert_table_bytes()) {
+                kernels::insert_lanes<view_type, StatusHandler, detail::keys_per_lane(), true>
+                    <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, view(),
+                                                                                 probing_length, status_out);
+                return;
+            }
+        }
+        if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+            if (detail::refill_inserts()) {
+                detail::launch_insert_refill<view_type, StatusHandler, detail::refill_insert_keys<unsigned(cg_size())>(), true>(
+                    keys_in, values_in, num_in, view(), probing_length, status_out, stream);
+                return;
+            }
+            if (detail::lockstep_inserts()) {
+                detail::launch_insert_lockstep<view_type, StatusHandler,
+                                               detail::lockstep_insert_keys<unsigned(cg_size())>(), true>(
+                    keys_in, values_in, num_in, view(), probing_length, status_out, stream);
+                return;
+            }
+        }
+        kernels::insert_batched<view_type, StatusHandler, detail::keys_per_group(), true>
+            <<<detail::grid_for_batched(num_in, cg_size()), block_size(), 0, stream>>>(
+                keys_in, values_in, num_in, view(), probing_length, status_out);
+    }
+
+    template<class StatusHandler>
+    WC_H void retrieve_aos(const key_type* keys_in, index_type num_in, value_type* values_out, hipStream_t stream,
+                           index_type probing_length, typename StatusHandler::base_type* status_out) const noexcept {
+        if constexpr (detail::LaneWindow<slot_type, cg_size()>::usable) {
+            if (capacity() * sizeof(slot_type) <= detail::lane_table_bytes()) {
+                kernels::retrieve_lanes<view_type, StatusHandler, detail::keys_per_lane()>
+                    <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, num_in, values_out, view(),
+                                                                                 probing_length, status_out);
+                return;
+            }
+        }
+        if constexpr (detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+            if (detail::refill_lookups()) {
+                detail::launch_retrieve_refill<view_type, StatusHandler, detail::refill_keys<unsigned(cg_size())>()>(
+                    keys_in, num_in, values_out, view(), probing_length, status_out, stream);
+                return;
+            }
+            if (detail::lockstep_lookups()) {
+                detail::launch_retrieve_lockstep<view_type, StatusHandler, detail::lockstep_keys<unsigned(cg_size())>()>(
+                    keys_in, num_in, values_out, view(), probing_length, status_out, stream);
+                return;
+            }
+        }
+        kernels::retrieve_batched<view_type, StatusHandler, detail::keys_per_group()>
+            <<<detail::grid_for_batched(num_in, cg_size()), block_size(), 0, stream>>>(
+                keys_in, num_in, values_out, view(), probing_length, status_out);
+    }
+
     WC_H void assign_status(status_type status, hipStream_t stream = 0) const noexcept {
         if (status_ != nullptr) {
             (void)hipMemcpyAsync(status_, &status, sizeof(status_type), hipMemcpyHostToDevice, stream);

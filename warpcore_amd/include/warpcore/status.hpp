@@ -1,10 +1,21 @@
 // Status flags and per-key status handlers.
 //
 // Same 13 flags, bit positions, error/warning masks and algebra as
-// include/warpcore/status.cuh:16-214 (flags 39-68, masks 71-86, atomic_join
-// 187-193), and the same three handlers (ReturnNothing 278-292, ReturnStatus
-// 296-312, ReturnBoolean 316-333). The bit layout is an interface: callers
-// read the words these tables write, so it is kept identical.
+// include/warpcore/status.cuh:16-214 (flags 39-68, masks 71-86, operator~
+// 146-150, atomic_join 187-193, mask 207), and the same three handlers
+// (ReturnNothing 278-292, ReturnStatus 296-312, ReturnBoolean 316-333). The
+// bit layout is an interface: callers read the words these tables write, so
+// it is kept identical, and so is the algebra, quirks included:
+//   - `mask` (what all() returns) is 0xFFF, so invalid_value (bit 12) is in
+//     neither all() nor warning_mask();
+//   - ~s is (~s) ^ 0xFFF: the low 12 bits keep their value and bits 12..31
+//     flip, so ~Status::none() is 0xFFFFF000;
+//   - therefore ReturnBoolean<Ignore> writes has_any(~Status(Ignore)): with
+//     the default Ignore = none it is true only for invalid_value, and an
+//     Ignore flag below bit 12 makes that flag count instead of be ignored.
+// ReturnAnyFlag<Ignore> (no reference counterpart) is the handler the
+// reference's comment describes: true iff any of the 13 flags outside
+// Ignore is set. Callers opt into it explicitly.
 #pragma once
 
 #include "base.hpp"
@@ -74,8 +85,8 @@ public:
     WC_HD constexpr bool has_max_values_for_key_reached() const noexcept { return has_any(max_values_for_key_reached()); }
     WC_HD constexpr bool has_invalid_value() const noexcept { return has_any(invalid_value()); }
 
-    // ~s flips every defined flag (status.cuh:146-150)
-    WC_HD constexpr Status operator~() const noexcept { return Status((~status_) & mask); }
+    // (~s) ^ mask (status.cuh:146-150): bits 0..11 unchanged, 12..31 flipped
+    WC_HD constexpr Status operator~() const noexcept { return Status((~status_) ^ mask); }
     WC_HD constexpr Status& operator+=(const Status& a) noexcept { status_ |= a.status_; return *this; }
     WC_HD constexpr Status operator+(const Status& a) const noexcept { return Status(status_ | a.status_); }
     WC_HD constexpr Status operator-(const Status& a) const noexcept { return Status(status_ & ~a.status_); }
@@ -94,11 +105,17 @@ public:
 
     WC_HD static constexpr Status from_base(base_type s) noexcept { return Status(s); }
 
+    // Every defined flag (bits 0..12) except those of `s`; unlike ~s this is
+    // the set-theoretic complement within the 13 flags. No reference
+    // counterpart.
+    WC_HD static constexpr Status flags_except(Status s) noexcept { return Status(defined_flags & ~s.status_); }
+
 private:
     WC_HD explicit constexpr Status(base_type s) noexcept : status_(s) {}
 
     static constexpr base_type one = 1;
-    static constexpr base_type mask = 0x1FFF;  // 13 defined flags
+    static constexpr base_type mask = 0xFFF;            // status.cuh:207
+    static constexpr base_type defined_flags = 0x1FFF;  // bits 0..12
 
     base_type status_;
 
@@ -151,7 +168,9 @@ public:
     }
 };
 
-// One bool per key: true if any flag outside `Ignore` is set (status.cuh:316-333).
+// One bool per key: status.has_any(~Status(Ignore)) with the reference's
+// operator~ (status.cuh:316-333), bit for bit. With Ignore = none only
+// invalid_value (and bits above the 13 flags) make it true.
 template<Status::base_type Ignore = 0>
 class ReturnBoolean {
 public:
@@ -159,6 +178,18 @@ public:
     using tag = tags::status_handler;
     WC_D static void handle(Status status, base_type* out, index_t index) noexcept {
         if (out != nullptr) out[index] = status.has_any(~Status(Ignore));
+    }
+};
+
+// One bool per key: true iff any of the 13 flags outside `Ignore` is set.
+// No reference counterpart; the opt-in alternative to ReturnBoolean<>.
+template<Status::base_type Ignore = 0>
+class ReturnAnyFlag {
+public:
+    using base_type = bool;
+    using tag = tags::status_handler;
+    WC_D static void handle(Status status, base_type* out, index_t index) noexcept {
+        if (out != nullptr) out[index] = status.has_any(Status::flags_except(Status::from_base(Ignore)));
     }
 };
 

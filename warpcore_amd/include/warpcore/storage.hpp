@@ -5,6 +5,9 @@
 // to their size so one lane reads a whole slot with one dwordx2/dwordx4 load
 // and a window of CG slots is one contiguous, line-aligned span; an 8-byte
 // u32/u32 slot is also the unit of the packed 64-bit compare-and-swap.
+// SoAStore keeps keys and values in two arrays (storage.cuh:218-432): a
+// window of CG keys is then CG * sizeof(Key) contiguous bytes and a value is
+// read only for the slot that holds the key.
 // CyclicStore is the ring of device temp words that host calls borrow for
 // counters (storage.cuh:15-148).
 #pragma once
@@ -87,6 +90,81 @@ template<class Key, class Value>
 struct alignas(detail::pair_alignment<Key, Value>()) pair_t {
     Key key;
     Value value;
+};
+
+// References to the key and value of one SoA slot (storage.cuh:189-210).
+template<class Key, class Value>
+struct pair_ref_t {
+    Key& key;
+    Value& value;
+};
+
+// Struct-of-arrays slots (storage.cuh:218-432).
+template<class Key, class Value>
+class SoAStore {
+public:
+    using key_type = Key;
+    using value_type = Value;
+    using slot_type = pair_t<Key, Value>;   // what a slot reads as (by value)
+    using status_type = Status;
+    using index_type = index_t;
+    using tag = tags::key_value_storage;
+
+    WC_H explicit SoAStore(index_type capacity) noexcept
+        : status_(Status::not_initialized()), capacity_(capacity), keys_(nullptr), values_(nullptr), is_copy_(false) {
+        if (capacity_ == 0) { status_ += Status::invalid_configuration(); return; }
+        const std::size_t bytes = (sizeof(Key) + sizeof(Value)) * capacity_;
+        if (warpcore::detail::available_gpu_memory() < bytes || hipMalloc(&keys_, sizeof(Key) * capacity_) != hipSuccess ||
+            hipMalloc(&values_, sizeof(Value) * capacity_) != hipSuccess) {
+            release();
+            status_ += Status::out_of_memory();
+            return;
+        }
+        status_ = Status::none();
+    }
+    WC_HD SoAStore(const SoAStore& o) noexcept
+        : status_(o.status_), capacity_(o.capacity_), keys_(o.keys_), values_(o.values_), is_copy_(true) {}
+    WC_H SoAStore(SoAStore&& o) noexcept
+        : status_(o.status_), capacity_(o.capacity_), keys_(o.keys_), values_(o.values_), is_copy_(o.is_copy_) {
+        o.is_copy_ = true;
+    }
+    WC_HD ~SoAStore() noexcept {
+#if !defined(__HIP_DEVICE_COMPILE__)
+        if (!is_copy_) release();
+#endif
+    }
+
+    WC_H void init_keys(key_type key, hipStream_t stream = 0) noexcept;
+    WC_H void init_values(value_type value, hipStream_t stream = 0) noexcept;
+    WC_H void init_pairs(key_type key, value_type value, hipStream_t stream = 0) noexcept {
+        init_keys(key, stream);
+        init_values(value, stream);
+    }
+
+    WC_D pair_ref_t<key_type, value_type> operator[](index_type i) noexcept { return {keys_[i], values_[i]}; }
+    WC_D pair_ref_t<const key_type, const value_type> operator[](index_type i) const noexcept {
+        return {keys_[i], values_[i]};
+    }
+
+    WC_HD key_type* keys() const noexcept { return keys_; }
+    WC_HD value_type* values() const noexcept { return values_; }
+    WC_HD status_type status() const noexcept { return status_; }
+    WC_HD index_type capacity() const noexcept { return capacity_; }
+    WC_H index_type bytes_total() const noexcept { return capacity_ * (sizeof(key_type) + sizeof(value_type)); }
+
+private:
+    WC_H void release() noexcept {
+        if (keys_ != nullptr) (void)hipFree(keys_);
+        if (values_ != nullptr) (void)hipFree(values_);
+        keys_ = nullptr;
+        values_ = nullptr;
+    }
+
+    status_type status_;
+    index_type capacity_;
+    key_type* keys_;
+    value_type* values_;
+    bool is_copy_;
 };
 
 // Array-of-structs slots (storage.cuh:438-645).
@@ -184,6 +262,18 @@ template<class Key, class Value>
 WC_H void AoSStore<Key, Value>::init_values(value_type value, hipStream_t stream) noexcept {
     if (status_.has_any() || capacity_ == 0) return;
     kernels::fill_values<<<warpcore::detail::grid_for(capacity_, 1, 8), block_size(), 0, stream>>>(pairs_, capacity_, value);
+}
+
+template<class Key, class Value>
+WC_H void SoAStore<Key, Value>::init_keys(key_type key, hipStream_t stream) noexcept {
+    if (status_.has_any() || capacity_ == 0) return;
+    kernels::fill<<<warpcore::detail::grid_for(capacity_, 1, 8), block_size(), 0, stream>>>(keys_, capacity_, key);
+}
+
+template<class Key, class Value>
+WC_H void SoAStore<Key, Value>::init_values(value_type value, hipStream_t stream) noexcept {
+    if (status_.has_any() || capacity_ == 0) return;
+    kernels::fill<<<warpcore::detail::grid_for(capacity_, 1, 8), block_size(), 0, stream>>>(values_, capacity_, value);
 }
 
 }  // namespace key_value

@@ -37,9 +37,21 @@ class Status(int):
     INVALID_PHASE_OVERLAP = 1 << 10
     MAX_VALUES_FOR_KEY_REACHED = 1 << 11
     INVALID_VALUE = 1 << 12
+    ALL = 0xFFF   # status.cuh:207: invalid_value is outside all()
 
-    def has_any(self, mask: int = 0x1FFF) -> bool:
+    def has_any(self, mask: int = ALL) -> bool:
         return bool(self & mask)
+
+    def __invert__(self) -> "Status":
+        """(~s) ^ 0xFFF as a 32-bit word (status.cuh:146-150)."""
+        return Status(((~int(self)) ^ self.ALL) & 0xFFFFFFFF)
+
+
+# Probing scheme, hasher and slot storage of SingleValueHashTable.custom
+# (include/warpcore/probing_schemes.cuh, hashers.cuh, storage.cuh).
+PROBING = {"double": 0, "linear": 1, "quadratic": 2}
+HASHER = {"murmur": 0, "mueller": 1, "nvidia": 2, "identity": 3}
+STORAGE = {"aos": 0, "soa": 1}
 
 
 def _ptr(t):
@@ -76,13 +88,21 @@ class SingleValueHashTable:
     _NAMES = {(32, 32): "u32u32", (64, 64): "u64u64", (32, 64): "u32u64", (64, 32): "u64u32"}
 
     def __init__(self, min_capacity: int, key_bits: int = 32, value_bits: int = 32,
-                 seed: int = DEFAULT_SEED, no_init: bool = False, cg_size: int = 0):
+                 seed: int = DEFAULT_SEED, no_init: bool = False, cg_size: int = 0,
+                 probing: str = "double", hasher: str = "murmur", storage: str = "aos"):
+        """probing / hasher / storage other than the defaults select
+        SingleValueHashTable<K, V, E, T, Probing<Hasher, Hasher, 8>, Storage>
+        (wc_svht_*_create_custom; u32u32 and u64u64 only)."""
         self.key_bits, self.value_bits = key_bits, value_bits
         self._p = f"wc_svht_{self._NAMES[(key_bits, value_bits)]}_"
         self._lib = _capi.lib()
-        self._h = self._fn("create")(min_capacity, seed, int(no_init), cg_size)
+        if (probing, hasher, storage) == ("double", "murmur", "aos"):
+            self._h = self._fn("create")(min_capacity, seed, int(no_init), cg_size)
+        else:
+            self._h = self._fn("create_custom")(min_capacity, seed, int(no_init), cg_size, PROBING[probing],
+                                                HASHER[hasher], HASHER[hasher], STORAGE[storage])
         if not self._h:
-            raise ValueError(f"unsupported configuration (cg_size={cg_size})")
+            raise ValueError(f"unsupported configuration (cg_size={cg_size}, {probing}, {hasher}, {storage})")
 
     def _fn(self, name):
         return getattr(self._lib, self._p + name)
@@ -134,6 +154,12 @@ class SingleValueHashTable:
         _check(self._fn("retrieve_all")(self._h, _ptr(keys_out), _ptr(values_out), ctypes.byref(n),
                                         _stream(stream)), "retrieve_all")
         return n.value
+
+    def for_each_count(self, counter_out, stream=None):
+        """for_each(f) with f adding one to counter_out[0] (a caller-zeroed
+        64-bit device word) per stored pair."""
+        _width(counter_out, 64, "counter_out")
+        _check(self._fn("for_each_count")(self._h, _ptr(counter_out), _stream(stream)), "for_each")
 
     def size(self, stream=None) -> int:
         return self._fn("size")(self._h, _stream(stream))
@@ -270,6 +296,11 @@ class HashSet:
     def erase(self, keys_in, stream=None, probing_length=DEFAULT_PROBING_LENGTH, status_out=None):
         _check(self._fn("erase")(self._h, _ptr(keys_in), keys_in.numel(), _stream(stream), probing_length,
                                  _ptr(status_out)), "erase")
+
+    def for_each_count(self, counter_out, stream=None):
+        """for_each(f) with f adding one to counter_out[0] per stored key."""
+        _width(counter_out, 64, "counter_out")
+        _check(self._fn("for_each_count")(self._h, _ptr(counter_out), _stream(stream)), "for_each")
 
     def retrieve_all(self, keys_out, stream=None) -> int:
         n = ctypes.c_uint64(0)
