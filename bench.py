@@ -68,12 +68,28 @@ def read_traffic(kernel_tag: str, log2n: int = 28):
     return float(m.group(1)) if m else None
 
 
+def probes_lib():
+    """ctypes handle of warpcore_amd/probes/libwarpcore_probes.so, prototypes
+    from its header."""
+    from warpcore_amd import _capi
+    d = REPO / "warpcore_amd" / "probes"
+    so = d / "libwarpcore_probes.so"
+    if not so.exists():
+        raise RuntimeError(f"{so} is missing: run the build first")
+    h = ctypes.CDLL(str(so))
+    for name, (restype, argtypes) in _capi.parse_header((d / "warpcore_probes.h").read_text()).items():
+        fn = getattr(h, name)
+        fn.restype, fn.argtypes = restype, argtypes
+    return h
+
+
 class Gpu:
     def __init__(self):
         import torch
         from warpcore_amd import _capi
         self.torch = torch
         self.lib = _capi.lib()
+        self.probes = probes_lib()
         self.stream = torch.cuda.Stream()
         self.sp = ctypes.c_void_p(self.stream.cuda_stream)
 
@@ -339,11 +355,11 @@ def run_line_roofline(g: Gpu, table_bytes: int, lines: int = 1 << 28, reps: int 
                 best = min(best, e0.elapsed_time(e1))
         return best
 
-    best = best_of(g.lib.wc_util_gather_lines, "gather")
+    best = best_of(g.probes.wc_util_gather_lines, "gather")
     res = {"glines_s": round(lines / (best * 1e-3) / 1e9, 2), "gbs": round(lines / (best * 1e-3) / 1e9 * LINE, 1),
            "footprint_bytes": table_bytes, "ms": round(best, 3)}
     if claim:   # each line read followed by a dependent CAS on one of its words: the insert's pattern
-        cbest = best_of(g.lib.wc_util_claim_lines, "claim")
+        cbest = best_of(g.probes.wc_util_claim_lines, "claim")
         res["claim_glines_s"] = round(lines / (cbest * 1e-3) / 1e9, 2)
     del buf, sink
     torch.cuda.empty_cache()
@@ -608,7 +624,7 @@ def main():
         b = torch.empty_like(a)
         a.fill_(1)
         torch.cuda.synchronize()
-        g.check(g.lib.wc_util_copy(g.p(b), g.p(a), a.numel(), None), "calibration copy")
+        g.check(g.probes.wc_util_copy(g.p(b), g.p(a), a.numel(), None), "calibration copy")
         torch.cuda.synchronize()
         log(f"[bench] calibration copy of {args.calibrate_copy_mib} MiB done")
 

@@ -254,6 +254,7 @@ WC_API uint32_t wc_counting_u64_peek_status(void* table, void* stream);
 WC_API uint32_t wc_counting_u64_pop_status(void* table, void* stream);
 WC_API uint32_t wc_counting_u64_cg_size(void* table);
 
+
 /* ------------------------------------------------------------------------
  * HashSet<K> (include/warpcore/hash_set.cuh)
  *   _create (min_capacity, seed)                                   :80-110
@@ -557,10 +558,15 @@ WC_API uint32_t wc_batch_u64u64_cg_size(void* batch);
  * arrival order (a step holds a run of lookups or one insert/erase, or one
  * chunk of it, per table). While fewer than 2^24 keys are pending (latency
  * mode), a request of at least direct_keys keys runs whole as a step of its
- * own on its table's bulk kernels (the kernels of one plain call). An
- * insert that does not fit its table's slot budget (max_load * capacity,
- * never given back) is refused: every key reports out_of_memory and wait
- * returns 0.
+ * own on its table's bulk kernels (the kernels of one plain call).
+ * Admission: each table has a budget of max_load * capacity claimed slots
+ * (stored keys and tombstones). An insert of n keys reserves n slots when
+ * admitted; the slots its steps did not claim (keys already stored) are
+ * given back when they have run. An insert that does not fit waits for the
+ * steps in flight; if it still does not fit, a table holding tombstones is
+ * compacted (rebuilt from its live pairs; results do not change), and an
+ * insert that does not fit even then is refused: every key reports
+ * out_of_memory and wait returns 0.
  *   create       table_capacities: HOST array (minimum capacities);
  *                max_requests per step (0 = 1024), item_keys per work item
  *                (0 = 64), max_step_keys (0 = 2^24), max_load (<= 0: 0.9),
@@ -568,11 +574,21 @@ WC_API uint32_t wc_batch_u64u64_cg_size(void* batch);
  *                every step on the step kernel)
  *   submit       device buffers, kept valid until the request is done;
  *                returns the request id (-1 on a bad argument)
- *   wait / drain block until one / every request is done
- *   latency_us   arrival to completion of a done request
+ *   wait         block until request id is done: 1 served, 0 refused,
+ *                -6 a step of the request failed to launch or complete
+ *                (its outputs are undefined), -7 unknown id (never issued,
+ *                or among the completions older than the last 2^20, whose
+ *                outcomes are dropped); -1 NULL server
+ *   drain        block until every request is done
+ *   latency_us   arrival to completion of a done request (-1 otherwise)
  *   step_of      the step a request ran in (1, 2, ...)
  *   stats        out3 = {requests done, steps, keys}
  *   direct_steps steps run on a table's bulk kernels
+ *   counters     out4 = {direct steps, compactions, failed requests, 0}
+ *   table_reserved / table_budget  claimed + reserved slots / the budget
+ *   fail_step    failure handling from the host (for tests): step number
+ *                `step` (1, 2, ... in issue order) is not launched and
+ *                reports a launch failure; its requests fail (-6)
  *   drive        replays num_requests requests (keys/values/outputs are
  *                slices of concatenated device arrays, sizes[i] each) at
  *                arrival_us[i] after the call starts; mode 0 through the
@@ -590,6 +606,10 @@ WC_API double wc_server_u64u64_latency_us(void* server, int64_t id);
 WC_API uint64_t wc_server_u64u64_step_of(void* server, int64_t id);
 WC_API int wc_server_u64u64_stats(void* server, uint64_t* out3);
 WC_API uint64_t wc_server_u64u64_direct_steps(void* server);
+WC_API int wc_server_u64u64_counters(void* server, uint64_t* out4);
+WC_API uint64_t wc_server_u64u64_table_reserved(void* server, uint32_t table);
+WC_API uint64_t wc_server_u64u64_table_budget(void* server, uint32_t table);
+WC_API int wc_server_u64u64_fail_step(void* server, uint64_t step);
 WC_API uint64_t wc_server_u64u64_table_size(void* server, uint32_t table);
 WC_API uint64_t wc_server_u64u64_table_capacity(void* server, uint32_t table);
 WC_API uint32_t wc_server_u64u64_table_status(void* server, uint32_t table);
@@ -605,6 +625,10 @@ WC_API double wc_server_u32u32_latency_us(void* server, int64_t id);
 WC_API uint64_t wc_server_u32u32_step_of(void* server, int64_t id);
 WC_API int wc_server_u32u32_stats(void* server, uint64_t* out3);
 WC_API uint64_t wc_server_u32u32_direct_steps(void* server);
+WC_API int wc_server_u32u32_counters(void* server, uint64_t* out4);
+WC_API uint64_t wc_server_u32u32_table_reserved(void* server, uint32_t table);
+WC_API uint64_t wc_server_u32u32_table_budget(void* server, uint32_t table);
+WC_API int wc_server_u32u32_fail_step(void* server, uint64_t step);
 WC_API uint64_t wc_server_u32u32_table_size(void* server, uint32_t table);
 WC_API uint64_t wc_server_u32u32_table_capacity(void* server, uint32_t table);
 WC_API uint32_t wc_server_u32u32_table_status(void* server, uint32_t table);
@@ -643,24 +667,6 @@ WC_API int64_t wc_load_binary_u32(const char* path, uint64_t max_keys, uint32_t*
 WC_API int64_t wc_load_binary_u64(const char* path, uint64_t max_keys, uint64_t* keys_out);
 WC_API int wc_load_keys_u32(const char* path, uint64_t max_keys, uint32_t* keys_d, uint64_t* num_out, void* stream);
 WC_API int wc_load_keys_u64(const char* path, uint64_t max_keys, uint64_t* keys_d, uint64_t* num_out, void* stream);
-
-/* Device copy of `bytes` (a multiple of 16, 16-byte aligned pointers), 16
- * bytes per lane: a streaming kernel of known traffic, used to calibrate
- * memory-traffic counters. No reference counterpart. */
-WC_API int wc_util_copy(void* dst, const void* src, uint64_t bytes, void* stream);
-/* Reads `lines` random 128-byte lines of the first table_bytes bytes at
- * `table` (16-byte aligned), 8 lanes per line and 8 lines in flight per
- * group: the rate the memory system serves random whole lines at this
- * footprint, the practical ceiling of the probing kernels. `sink` is a
- * device word the kernel may write. No reference counterpart. */
-WC_API int wc_util_gather_lines(const void* table, uint64_t table_bytes, uint64_t lines, uint64_t seed, uint32_t* sink, void* stream);
-/* As wc_util_gather_lines, and then one lane per line replaces the 8 bytes
- * it read by their value plus one with a compare-and-swap that returns the
- * old word (the lane is chosen by the data read, so the CAS depends on the
- * line): the rate of "read a random line, claim a word of it", the memory
- * pattern of an insert without the probing. Writes into `table`. No
- * reference counterpart. */
-WC_API int wc_util_claim_lines(void* table, uint64_t table_bytes, uint64_t lines, uint64_t seed, uint32_t* sink, void* stream);
 
 /* Capacity rule detail::get_valid_capacity (include/warpcore/base.cuh:27-34)
  * and the unique-key map, evaluated on the host. */

@@ -203,6 +203,13 @@ __global__ void store_cas(Store st, Key init, Key key, Value value, unsigned lon
     if (!(st[0].key == key && st[0].value == value)) atomicAdd(errors, 1ull);
 }
 
+// Sets the count of `key` (a slot of the counting table) to `count`.
+template<class View>
+__global__ void counting_preset(View v, typename View::key_type key, typename View::value_type count) {
+    const index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < v.capacity() && v.base.slots[i].key == key) v.base.slots[i].value = count;
+}
+
 template<class Store>
 int store_check(std::uint64_t capacity, unsigned long long* errors) {
     using Key = typename Store::key_type;
@@ -339,6 +346,28 @@ int wcx_store_check(int kind, std::uint64_t capacity, unsigned long long* errors
     case 3: return store_check<SoAStore<std::uint32_t, float>>(capacity, errors);
     default: return -1;
     }
+}
+
+// CountingHashTable<u32, u32 counts> with max_count: key keys[0] is inserted
+// once and its count set to `preset`, then the whole batch keys[0..n) is
+// inserted by the host bulk call (combined counts for n >= 4096). Writes
+// the per-occurrence statuses, the counts of keys[0..n) and the table
+// status word.
+int wcx_counting_u32_saturation(const std::uint32_t* keys, std::uint64_t n, std::uint32_t max_count,
+                                std::uint32_t preset, std::uint32_t* status_out, std::uint32_t* counts_out,
+                                std::uint32_t* table_status) {
+    using table_t = CountingHashTable<std::uint32_t, std::uint32_t>;
+    table_t table(2 * n + 1024, defaults::seed<std::uint32_t>(), max_count);
+    table.insert(keys, 1);
+    if (hipDeviceSynchronize() != hipSuccess) return -1;
+    std::uint32_t first = 0;
+    if (hipMemcpy(&first, keys, sizeof(first), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    counting_preset<<<blocks_for(table.capacity()), 256>>>(table.view(), first, preset);
+    table.template insert<status_handlers::ReturnStatus>(keys, n, 0, defaults::probing_length(),
+                                                          reinterpret_cast<Status*>(status_out));
+    table.retrieve(keys, n, counts_out);
+    *table_status = table.peek_status().base();
+    return int(hipDeviceSynchronize());
 }
 
 }  // extern "C"

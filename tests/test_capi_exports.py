@@ -30,3 +30,17 @@ def test_binding_loads():
     _capi.lib.cache_clear()
     lib = _capi.lib()
     assert lib.wc_svht_u32u32_create.restype is ctypes.c_void_p
+
+
+def test_probes_library_exports_its_header():
+    """The benchmark's measurement probes live in their own library, not in
+    the table library's C ABI."""
+    from warpcore_amd.build import PROBES, PROBES_LIB, build_probes
+    if not PROBES_LIB.exists():
+        build_probes()
+    protos = _capi.parse_header((PROBES / "warpcore_probes.h").read_text())
+    assert set(protos) == {"wc_util_copy", "wc_util_gather_lines", "wc_util_claim_lines"}
+    lib = ctypes.CDLL(str(PROBES_LIB))
+    assert all(hasattr(lib, n) for n in protos)
+    assert not any(n.startswith("wc_util_") for n in _capi.parse_header())
+

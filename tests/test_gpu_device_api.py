@@ -159,3 +159,32 @@ def test_multi_value_device_roundtrip(dlib, kind):
         exp.setdefault(key, []).append(val)
     for i, key in enumerate(keys.tolist()):
         assert sorted(got[i, : mult[i]].tolist()) == sorted(exp[key])
+
+
+@pytest.mark.parametrize("max_count,preset", [((1 << 32) - 1, (1 << 32) - 10), (1000, 990), (1000, 1000)])
+def test_counting_combined_add_saturates(dlib, max_count, preset):
+    """A combined count (one add of all occurrences of a key in a tile) ends
+    where single increments would (counting_hash_table.cuh:154-170: each adds
+    one while the count is below max_count): min(preset + occurrences,
+    max_count), with no index_overflow even where one add would pass 2^32-1."""
+    dlib.wcx_counting_u32_saturation.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32,
+                                                 ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                                                 ctypes.c_void_p]
+    rng = np.random.default_rng(31)
+    others = unique_keys(8192, 32, seed=32)
+    hot = others[0]
+    batch = np.concatenate([np.full(40, hot, np.uint32), others[1:]])
+    rng.shuffle(batch)
+    batch = np.concatenate([[hot], batch]).astype(np.uint32)   # keys[0] is the preset key
+    n = len(batch)
+    st, counts = dev_empty(n, 32, 0x77), dev_empty(n, 32, 0x55)
+    word = ctypes.c_uint32(0)
+    assert dlib.wcx_counting_u32_saturation(_p(to_dev(batch)), n, max_count, preset, _p(st), _p(counts),
+                                            ctypes.byref(word)) == 0
+    got_s, got_c = from_dev(st, np.uint32), from_dev(counts, np.uint32)
+    is_hot = batch == hot
+    assert np.all(got_s[is_hot] == S_DUP)            # the key was in the table before the batch
+    assert np.all(got_s[~is_hot] == S_NONE)
+    assert np.all(got_c[is_hot] == min(preset + int(is_hot.sum()), max_count))
+    assert np.all(got_c[~is_hot] == 1)
+    assert word.value == S_DUP                        # duplicate_key joined, no index_overflow (64)

@@ -166,3 +166,189 @@ def test_server_concurrent_clients(cuda):
             assert np.all(got_s[hit] == S_NONE) and np.all(got_s[~hit] == S_NOT_FOUND)
             np.testing.assert_array_equal(got_o[hit], qn[hit] ^ np.uint64(0xABCD))
     assert srv.stats()["requests"] == 16
+
+
+def _pair_multiset(keys, st):
+    order = np.lexsort((st, keys))
+    return keys[order], st[order]
+
+
+def test_server_matches_oracle_per_table(cuda, oracle):
+    """A served stream equals the oracle fed each table's requests in
+    arrival order (the server's per-table linearised order): every lookup's
+    values and statuses, every erase's statuses, every insert's multiset of
+    (key, status) pairs (which duplicate of a key reports `none` is free),
+    the final sizes and status words. 8 tables of 2^20 slots, requests of
+    2^8..2^18 keys arriving as a Poisson stream."""
+    print("[serving] oracle replay: 8 tables, 160 requests of 2^8..2^18 keys", flush=True)
+    caps = np.full(8, 1 << 20, np.uint64)
+    ops, tables, sizes, arrival, keys, vals = _stream(160, len(caps), seed=17, lo=8, hi=18, universe=1 << 18)
+    out_s, st_s, lat_s, mk_s, size_s, tst_s, stats = _run(0, caps, ops, tables, sizes, arrival, keys, vals)
+    assert stats["requests"] == len(ops) and stats["failed"] == 0
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    models = [oracle.SVHT(int(c), 64, 64, cg=8) for c in caps]
+    for r in range(len(ops)):
+        s = slice(off[r], off[r + 1])
+        o = models[tables[r]]
+        if ops[r] == INSERT:
+            exp = o.insert(keys[r], vals[r])
+            gk, gs = _pair_multiset(keys[r], st_s[s])
+            ek, es = _pair_multiset(keys[r], exp)
+            np.testing.assert_array_equal(gs, es, err_msg=f"insert request {r}")
+        elif ops[r] == RETRIEVE:
+            ev, es = o.retrieve(keys[r])
+            np.testing.assert_array_equal(st_s[s], es, err_msg=f"retrieve request {r}")
+            hit = es == S_NONE
+            np.testing.assert_array_equal(out_s[s][hit], ev[hit], err_msg=f"retrieve request {r}")
+            assert np.all(out_s[s][~hit] == np.uint64(0x5A5A)), f"request {r}: a miss wrote its output"
+        else:
+            es = o.erase(keys[r])
+            np.testing.assert_array_equal(st_s[s], es, err_msg=f"erase request {r}")
+    assert size_s == [m.size() for m in models]
+    assert tst_s == [m.status() for m in models]
+
+
+def test_server_config_shape_matches_launch_per_request(cuda):
+    """The serving configuration's shape: 64 resident SingleValueHashTable
+    <u64,u64> of 97.66 M slots (100 GB), 2,000 requests of 2^8..2^22 keys
+    (log-uniform) arriving as a Poisson stream, 30 % insert / 60 % retrieve /
+    10 % erase, keys from a 2^28-key universe with 20 % Zipf(1.0) hot keys.
+    The server and one plain launch per request (each on its own 100 GB of
+    tables) must agree on every lookup's value and status, every erase's
+    status, every insert's count of `none` statuses and the sum of the keys
+    reporting it, and every table's size and status word."""
+    from warpcore_amd import _capi
+    from warpcore_amd.tables import Server
+    lib = _capi.lib()
+    num_tables, num_requests = 64, 2000
+    rng = np.random.default_rng(0xC5)
+    caps = np.full(num_tables, int(100e9 / 16 / num_tables), np.uint64)
+    sizes = np.exp2(rng.uniform(8, 22, num_requests)).astype(np.uint64)
+    ops = rng.choice([INSERT, RETRIEVE, ERASE], size=num_requests, p=[0.3, 0.6, 0.1]).astype(np.uint32)
+    tabs = rng.integers(0, num_tables, size=num_requests).astype(np.uint32)
+    n = int(sizes.sum())
+    print(f"[serving] config shape: {num_tables} tables x {int(caps[0])} slots, {num_requests} requests, "
+          f"{n} keys", flush=True)
+    U = 1 << 28
+    universe = torch.empty(U, dtype=torch.int64, device="cuda")
+    assert lib.wc_gen_unique_u64(universe.data_ptr(), U, 1, None) == 0
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0xC5)
+    idx = torch.randint(0, U, (n,), device="cuda", generator=g)
+    # hot keys: Zipf(1.0) ranks k over the first 2^20 universe keys (universe[k - 1])
+    hot = torch.empty(n, dtype=torch.int64, device="cuda")
+    assert lib.wc_gen_zipf_u64(hot.data_ptr(), n, 1 << 20, 1.0, 0xC6, 1, None) == 0
+    take = torch.rand(n, device="cuda", generator=g) < 0.2
+    keys = torch.where(take, hot, universe[idx])
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    for r in np.flatnonzero(ops == ERASE):   # distinct keys within an erase: a contiguous universe range
+        m = int(sizes[r])
+        start = int(rng.integers(0, U - m))
+        keys[off[r]:off[r + 1]] = universe[start:start + m]
+    del universe, hot, take, idx
+    req = torch.repeat_interleave(torch.arange(num_requests, device="cuda"),
+                                  torch.from_numpy(sizes.astype(np.int64)).to("cuda"))
+    values = keys * 3 + req          # one value per (key, request): duplicates in an insert agree
+    rate = 0.5 * num_requests / 150e3   # about half the launch-per-request throughput
+    arrival = np.cumsum(rng.exponential(1.0 / rate, size=num_requests))
+    torch.cuda.synchronize()
+
+    def run(mode):
+        srv = Server(caps, 64, 64)
+        out = torch.full((n,), 0x5A5A, dtype=torch.int64, device="cuda")
+        st = torch.full((n,), 0x77, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        srv.drive(mode, ops, tabs, sizes, arrival, keys, values, out, st)
+        torch.cuda.synchronize()
+        sizes_t = [srv.table_size(t) for t in range(num_tables)]
+        status_t = [int(srv.table_status(t)) for t in range(num_tables)]
+        stats = srv.stats() if mode == 0 else None
+        del srv
+        torch.cuda.empty_cache()
+        return out, st, sizes_t, status_t, stats
+
+    out_s, st_s, size_s, tst_s, stats = run(0)
+    print(f"[serving] server: {stats}", flush=True)
+    out_p, st_p, size_p, tst_p, _ = run(1)
+    assert stats["requests"] == num_requests and stats["failed"] == 0
+    op_of = torch.from_numpy(ops.astype(np.int64)).to("cuda")[req]
+    look, ins, era = op_of == RETRIEVE, op_of == INSERT, op_of == ERASE
+    assert torch.equal(st_s[look], st_p[look]) and torch.equal(out_s[look], out_p[look])
+    assert torch.equal(st_s[era], st_p[era])
+    hits = int((st_s[look] == S_NONE).sum())
+    assert hits > 0.05 * int(look.sum())         # the stream really looks up stored keys
+    for st in (st_s, st_p):
+        assert bool(((st[ins] == S_NONE) | (st[ins] == S_DUP)).all())
+
+    def none_by_request(st):
+        first = (st == S_NONE) & ins
+        cnt = torch.zeros(num_requests, dtype=torch.int64, device="cuda").index_add_(0, req, first.long())
+        ksum = torch.zeros(num_requests, dtype=torch.int64, device="cuda").index_add_(0, req,
+                                                                                      torch.where(first, keys, 0))
+        return cnt, ksum
+
+    cs, ks = none_by_request(st_s)
+    cp, kp = none_by_request(st_p)
+    assert torch.equal(cs, cp) and torch.equal(ks, kp)
+    assert size_s == size_p and tst_s == tst_p
+
+
+def test_server_insert_erase_cycles_stay_admitted(cuda):
+    """Erase leaves tombstones that keep their slots; an insert that no
+    longer fits the budget has its table compacted (live pairs re-inserted)
+    and is admitted. Repeated inserts of stored keys claim nothing: their
+    reservation is given back, and one that does not fit the budget is
+    admitted after a lookup shows its keys are stored. Each round: lookups
+    see exactly the live keys."""
+    from warpcore_amd.tables import Server
+    srv = Server([1 << 17], 64, 64)
+    budget = srv.table_budget(0)
+    n = int(budget * 0.6)
+    live = None
+    for rnd in range(4):
+        print(f"[serving] cycle {rnd}", flush=True)
+        k = unique_keys(n, 64, seed=200 + rnd)
+        kd, vd = to_dev(k), to_dev(k ^ np.uint64(rnd + 1))
+        for rep in range(3):   # the first insert claims n slots, the repeats none
+            st = dev_empty(n, 32, 0x77)
+            torch.cuda.synchronize()
+            assert srv.wait(srv.submit(INSERT, 0, kd, vd, None, st)), f"round {rnd} insert {rep} refused"
+            assert np.all(from_dev(st, np.uint32) == (S_NONE if rep == 0 else S_DUP))
+        assert srv.table_reserved(0) <= budget
+        q = np.concatenate([k, live]) if live is not None else k
+        qd = to_dev(q)
+        out, qs = dev_empty(len(q), 64, 0x1), dev_empty(len(q), 32)
+        torch.cuda.synchronize()
+        assert srv.wait(srv.submit(RETRIEVE, 0, qd, None, out, qs))
+        got_s = from_dev(qs, np.uint32)
+        assert np.all(got_s[:n] == S_NONE) and np.all(got_s[n:] == S_NOT_FOUND)
+        np.testing.assert_array_equal(from_dev(out, np.uint64)[:n], k ^ np.uint64(rnd + 1))
+        es = dev_empty(n, 32, 0x77)
+        torch.cuda.synchronize()
+        assert srv.wait(srv.submit(ERASE, 0, kd, None, None, es))
+        assert np.all(from_dev(es, np.uint32) == S_NONE)
+        live = k
+    assert srv.stats()["compactions"] >= 2
+    assert srv.table_size(0) == 0
+
+
+def test_server_failed_step_fails_its_requests(cuda):
+    """Failure handling forced from the host: the scheduler's second step
+    is reported as a launch failure without running. wait() returns the
+    failure code for its request; the other requests are served, and the
+    table holds only what the served inserts stored."""
+    from warpcore_amd.tables import Server
+    srv = Server([1 << 18], 64, 64)
+    srv.fail_step(2)
+    batches = [unique_keys(1000, 64, seed=300 + i) for i in range(3)]
+    codes = []
+    for b in batches:
+        kd = to_dev(b)
+        st = dev_empty(len(b), 32)
+        torch.cuda.synchronize()
+        rid = srv.submit(INSERT, 0, kd, kd, None, st)
+        codes.append(srv.wait_code(rid))   # one request per step: waited before the next
+    assert codes == [Server.SERVED, Server.FAILED, Server.SERVED]
+    assert srv.stats()["failed"] == 1
+    assert srv.table_size(0) == 2000
+    assert srv.wait_code(10 ** 9) == Server.UNKNOWN_ID

@@ -140,3 +140,80 @@ def test_table_batch_tuner_keeps_results(cuda):
             exp = np.array([lookup.get(x, 0xAB) for x in q.tolist()], np.uint32)
             np.testing.assert_array_equal(got[sl], exp)
             np.testing.assert_array_equal(gq[sl], np.where(np.isin(q, keys[t]), S_NONE, S_NOT_FOUND))
+
+
+def test_table_batch_full_configuration(cuda, oracle):
+    """The batched configuration at full size: 65,536 SingleValueHashTable
+    <u32,u32> of capacity log-uniform in 2^6..2^16 slots, each filled to a
+    load factor uniform in [0.5, 0.9] (about 433 M keys). Size-independent
+    properties over every table: all inserts report none, every stored key
+    returns its value, every absent key reports key_not_found and leaves its
+    output untouched, and the per-table status words are exact; plus the
+    oracle on 48 tables spread over the whole capacity range (the largest
+    ones, of 2^15..2^16 slots, included)."""
+    from warpcore_amd import _capi
+    from warpcore_amd.tables import TableBatch
+    lib = _capi.lib()
+    num_tables = 65536
+    rng = np.random.default_rng(0xC7)
+    caps = np.exp2(rng.uniform(6, 16, num_tables)).astype(np.uint64)
+    n_t = np.maximum((caps * rng.uniform(0.5, 0.9, num_tables)).astype(np.uint64), 1)
+    off = np.concatenate([[0], np.cumsum(n_t)]).astype(np.uint64)
+    n = int(off[-1])
+    print(f"[batch] full configuration: {num_tables} tables, {n} keys", flush=True)
+    keys = torch.empty(n, dtype=torch.int32, device="cuda")
+    absent = torch.empty(n, dtype=torch.int32, device="cuda")
+    values = torch.empty(n, dtype=torch.int32, device="cuda")
+    # fmix32 is a bijection: offsets 0x7000 + [0, n) and 0x7000 + [n, 2n) give disjoint key sets
+    assert lib.wc_gen_unique_u32(keys.data_ptr(), n, 0x7000, None) == 0
+    assert lib.wc_gen_unique_u32(absent.data_ptr(), n, 0x7000 + n, None) == 0
+    assert lib.wc_gen_uniform_u32(values.data_ptr(), n, 0xC7, None) == 0
+    offs = to_dev(off)
+    b = TableBatch(caps, 32, 32, cg_size=16)
+    st = torch.full((n,), 0x77, dtype=torch.int32, device="cuda")
+    b.insert(keys, values, offs, status_out=st)
+    assert bool((st == S_NONE).all())
+    assert bool((b.status_words() == S_NONE).all())
+    out = torch.full((n,), 0xAB, dtype=torch.int32, device="cuda")
+    qst = torch.full((n,), 0x77, dtype=torch.int32, device="cuda")
+    b.retrieve(keys, offs, out, status_out=qst)
+    assert bool((qst == S_NONE).all()) and torch.equal(out, values)
+    miss = torch.full((n,), 0xAB, dtype=torch.int32, device="cuda")
+    b.retrieve(absent, offs, miss, status_out=qst)
+    assert bool((qst == S_NOT_FOUND).all()) and bool((miss == 0xAB).all())
+    assert bool((b.status_words() == S_NOT_FOUND).all())
+    torch.cuda.synchronize()
+    del miss, qst, st
+    # oracle per table on a spread of capacities, the largest included
+    order = np.argsort(caps, kind="stable")
+    sample = np.unique(np.concatenate([order[np.linspace(0, num_tables - 1, 40).astype(np.int64)], order[-8:]]))
+    assert caps[sample].max() >= (1 << 15)
+    b.init()
+    st = torch.full((n,), 0x77, dtype=torch.int32, device="cuda")
+    b.insert(keys, values, offs, status_out=st)
+    # queries of table t: its keys, then its absent keys (two lookups over
+    # the same offsets)
+    out1 = torch.full((n,), 0xAB, dtype=torch.int32, device="cuda")
+    out2 = torch.full((n,), 0xAB, dtype=torch.int32, device="cuda")
+    s1 = torch.full((n,), 0x77, dtype=torch.int32, device="cuda")
+    s2 = torch.full((n,), 0x77, dtype=torch.int32, device="cuda")
+    b.retrieve(keys, offs, out1, status_out=s1)
+    b.retrieve(absent, offs, out2, status_out=s2)
+    torch.cuda.synchronize()
+    words = from_dev(b.status_words(), np.uint32)
+    for t in sample.tolist():
+        lo, hi = int(off[t]), int(off[t + 1])
+        k = from_dev(keys[lo:hi], np.uint32)
+        a = from_dev(absent[lo:hi], np.uint32)
+        v = from_dev(values[lo:hi], np.uint32)
+        o = oracle.SVHT(int(caps[t]), 32, 32, cg=16)
+        np.testing.assert_array_equal(from_dev(st[lo:hi], np.uint32), o.insert(k, v), err_msg=f"table {t}")
+        q = np.concatenate([k, a])
+        ov, ost = o.retrieve(q)
+        gs = np.concatenate([from_dev(s1[lo:hi], np.uint32), from_dev(s2[lo:hi], np.uint32)])
+        gv = np.concatenate([from_dev(out1[lo:hi], np.uint32), from_dev(out2[lo:hi], np.uint32)])
+        np.testing.assert_array_equal(gs, ost, err_msg=f"table {t}")
+        hit = ost == S_NONE
+        np.testing.assert_array_equal(gv[hit], ov[hit], err_msg=f"table {t}")
+        assert np.all(gv[~hit] == 0xAB)
+        assert words[t] == o.status(), f"table {t}"

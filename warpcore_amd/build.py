@@ -2,7 +2,9 @@
 
 Every ``csrc/*.hip`` translation unit is compiled with hipcc for gfx950 in
 parallel and linked into ``warpcore_amd/libwarpcore_amd.so``. Object files
-go to ``warpcore_amd/_build/``. A build always recompiles.
+go to ``warpcore_amd/_build/``. The measurement probes of the benchmark
+(``probes/*.hip``) go into their own ``warpcore_amd/probes/
+libwarpcore_probes.so``. A build always recompiles.
 """
 from __future__ import annotations
 
@@ -18,6 +20,8 @@ CSRC = PKG / "csrc"
 INCLUDE = PKG / "include"
 BUILD = PKG / "_build"
 LIB = PKG / "libwarpcore_amd.so"
+PROBES = PKG / "probes"
+PROBES_LIB = PROBES / "libwarpcore_probes.so"
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
@@ -67,6 +71,18 @@ def build(jobs: int | None = None, verbose: bool = False) -> Path:
     return LIB
 
 
+def build_probes() -> Path:
+    """Compile the benchmark's measurement probes into their own library."""
+    tmp = PROBES_LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, *HIP_FLAGS, "-shared", *map(str, sorted(PROBES.glob("*.hip"))), "-o", str(tmp)]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"hipcc failed for the probes:\n{proc.stdout}{proc.stderr}")
+    os.replace(tmp, PROBES_LIB)
+    return PROBES_LIB
+
+
 if __name__ == "__main__":
     path = build(verbose="-v" in sys.argv)
     print(path)
+    print(build_probes())

@@ -141,8 +141,9 @@ int drive(S* s, int mode, const std::uint32_t* ops, const std::uint32_t* tables,
         } catch (...) { return -1; }                                                                              \
     }                                                                                                             \
     extern "C" int wc_server_##NAME##_wait(void* h, int64_t id) {                                                 \
-        if (!h || id < 0) return wc_capi::kBadHandle;                                                             \
-        return static_cast<Srv_##NAME*>(h)->wait(index_t(id)) ? 1 : 0;                                           \
+        if (!h) return wc_capi::kBadHandle;                                                                       \
+        if (id < 0) return serving::kUnknownId;                                                                   \
+        return static_cast<Srv_##NAME*>(h)->wait(index_t(id));                                                    \
     }                                                                                                             \
     extern "C" int wc_server_##NAME##_drain(void* h) {                                                            \
         if (!h) return wc_capi::kBadHandle;                                                                       \
@@ -165,6 +166,28 @@ int drive(S* s, int mode, const std::uint32_t* ops, const std::uint32_t* tables,
     }                                                                                                             \
     extern "C" uint64_t wc_server_##NAME##_direct_steps(void* h) {                                                \
         return h ? static_cast<Srv_##NAME*>(h)->stats().direct_steps : 0;                                        \
+    }                                                                                                             \
+    extern "C" int wc_server_##NAME##_counters(void* h, uint64_t* out4) {                                         \
+        if (!h || !out4) return wc_capi::kBadHandle;                                                              \
+        const auto s = static_cast<Srv_##NAME*>(h)->stats();                                                      \
+        out4[0] = s.direct_steps;                                                                                 \
+        out4[1] = s.compactions;                                                                                  \
+        out4[2] = s.failed;                                                                                       \
+        out4[3] = 0;                                                                                              \
+        return 0;                                                                                                 \
+    }                                                                                                             \
+    extern "C" uint64_t wc_server_##NAME##_table_reserved(void* h, uint32_t table) {                              \
+        if (!h || table >= static_cast<Srv_##NAME*>(h)->num_tables()) return 0;                                   \
+        return static_cast<Srv_##NAME*>(h)->reserved(table);                                                      \
+    }                                                                                                             \
+    extern "C" int wc_server_##NAME##_fail_step(void* h, uint64_t step) {                                         \
+        if (!h) return wc_capi::kBadHandle;                                                                       \
+        static_cast<Srv_##NAME*>(h)->fail_step(step);                                                             \
+        return 0;                                                                                                 \
+    }                                                                                                             \
+    extern "C" uint64_t wc_server_##NAME##_table_budget(void* h, uint32_t table) {                                \
+        if (!h || table >= static_cast<Srv_##NAME*>(h)->num_tables()) return 0;                                   \
+        return static_cast<Srv_##NAME*>(h)->budget(table);                                                        \
     }                                                                                                             \
     extern "C" uint64_t wc_server_##NAME##_table_size(void* h, uint32_t table) {                                  \
         if (!h || table >= static_cast<Srv_##NAME*>(h)->num_tables()) return 0;                                   \

@@ -59,6 +59,12 @@ struct CountingView {
     // check before the add and the wrap guard after it can only act after
     // 2^64 increments of one key, so the add is issued without reading the
     // count and without waiting for the old value.
+    //
+    // A combined increment (AddValue: `inc` occurrences of the key at once)
+    // ends where `inc` single increments would: each of those adds one only
+    // while the count is below max_count, so the count becomes
+    // min(count + inc, max_count) and never wraps (no index_overflow, which
+    // a single add of `inc` past the type's maximum would raise).
     template<class Join>
     WC_D void finish_insert(index_type slot, const value_type& inc, bool, Status& status, Join& join) const noexcept {
         const value_type add = AddValue ? inc : value_type(1);
@@ -68,6 +74,16 @@ struct CountingView {
                 __hip_atomic_fetch_add(v, add, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 return;
             }
+        }
+        if constexpr (AddValue) {
+            value_type cur = atomic_load_relaxed(v);
+            while (cur < max_count) {
+                const value_type want = max_count - cur < add ? max_count : value_type(cur + add);
+                const value_type prev = atomic_cas(v, cur, want);
+                if (prev == cur) break;
+                cur = prev;
+            }
+            return;
         }
         if (atomic_load_relaxed(v) < max_count) {
             const value_type old = atomic_add(v, add);
@@ -377,6 +393,46 @@ public:
                 keys_in, values_in, num_in, v, probing_length, status_out);
     }
 
+    // Buffers of one combining pass over up to `capacity` occurrences, carved
+    // from one device buffer: distinct keys, their counts, first occurrences,
+    // each occurrence's entry, the entries' statuses and the entry count.
+    struct CombineBuffers {
+        key_type* keys = nullptr;
+        value_type* counts = nullptr;
+        std::uint32_t* rep = nullptr;
+        std::uint32_t* pos = nullptr;
+        Status* status = nullptr;
+        index_type* num = nullptr;
+        index_type capacity = 0;
+
+        WC_HD static constexpr index_type align(index_type b) noexcept { return (b + 255) / 256 * 256; }
+        // bytes of a buffer for `n` occurrences
+        WC_HD static constexpr index_type bytes_for(index_type n) noexcept {
+            return align(sizeof(key_type) * n) + align(sizeof(value_type) * n) + 3 * align(sizeof(std::uint32_t) * n) +
+                   align(sizeof(index_type));
+        }
+        // the buffers inside `base` (256-byte aligned, bytes_for(n) bytes)
+        WC_HD static CombineBuffers carve(void* base, index_type n) noexcept {
+            auto* p = static_cast<unsigned char*>(base);
+            CombineBuffers b;
+            b.keys = reinterpret_cast<key_type*>(p);
+            p += align(sizeof(key_type) * n);
+            b.counts = reinterpret_cast<value_type*>(p);
+            p += align(sizeof(value_type) * n);
+            b.rep = reinterpret_cast<std::uint32_t*>(p);
+            p += align(sizeof(std::uint32_t) * n);
+            b.pos = reinterpret_cast<std::uint32_t*>(p);
+            p += align(sizeof(std::uint32_t) * n);
+            b.status = reinterpret_cast<Status*>(p);
+            p += align(sizeof(std::uint32_t) * n);
+            b.num = reinterpret_cast<index_type*>(p);
+            b.capacity = n;
+            return b;
+        }
+    };
+
+    // Combining passes in chunks of the table's own buffers (grown on
+    // demand); each pass reads its entry count back to size the insert.
     template<class StatusHandler>
     WC_H void insert_combined(const key_type* keys_in, index_type num_in, hipStream_t stream,
                               index_type probing_length, typename StatusHandler::base_type* status_out) noexcept {
@@ -387,29 +443,65 @@ public:
                                                          status_out);
             return;
         }
+        const CombineBuffers buf = scratch_.buffers();
         for (index_type first = 0; first < num_in; first += cap) {
             const index_type n = num_in - first < cap ? num_in - first : cap;
-            storage::kernels::fill<index_type><<<1, 64, 0, stream>>>(scratch_.num, 1, index_type(0));
-            const index_type tiles = SDIV(n, index_type(detail::kCombineTile));
-            const index_type limit = index_type(detail::num_compute_units()) * 4;
-            kernels::count_combine<key_type, value_type, EmptyKey, TombstoneKey>
-                <<<unsigned(tiles < limit ? tiles : limit), 256, 0, stream>>>(
-                    keys_in + first, n, scratch_.keys, scratch_.counts, scratch_.rep, scratch_.pos, scratch_.num);
+            combine(keys_in + first, n, buf, stream);
             index_type entries = 0;
-            (void)hipMemcpyAsync(&entries, scratch_.num, sizeof(index_type), hipMemcpyDeviceToHost, stream);
+            (void)hipMemcpyAsync(&entries, buf.num, sizeof(index_type), hipMemcpyDeviceToHost, stream);
             (void)hipStreamSynchronize(stream);
             insert_each<add_view_type, status_handlers::ReturnStatus, true>(
-                scratch_.keys, scratch_.counts, entries, add_view(), stream, probing_length, scratch_.status);
-            kernels::count_join_duplicates<value_type>
-                <<<detail::grid_for(entries, 1, 4), block_size(), 0, stream>>>(scratch_.counts, scratch_.status,
-                                                                            scratch_.num, base_table_.status_);
-            if constexpr (!std::is_same<StatusHandler, status_handlers::ReturnNothing>::value)
-                if (status_out != nullptr)
-                    kernels::count_expand<StatusHandler><<<detail::grid_for(n, 1, 8), block_size(), 0, stream>>>(
-                        scratch_.pos, scratch_.rep, scratch_.status, first, n, status_out);
+                buf.keys, buf.counts, entries, add_view(), stream, probing_length, buf.status);
+            finish_combined<StatusHandler>(buf, first, n, entries, stream, status_out);
         }
     }
 
+    // One combining pass over num_in <= buf.capacity occurrences in
+    // caller-provided buffers, with no host round trip: the insert reads the
+    // entry count on the device (refill kernel, sized by num_in). Results,
+    // counts and statuses are those of insert(). Requires a fixed-step
+    // probing scheme.
+    template<class StatusHandler = defaults::status_handler_t>
+    WC_H void insert_combined_async(const key_type* keys_in, index_type num_in, const CombineBuffers& buf,
+                                    hipStream_t stream = 0, index_type probing_length = defaults::probing_length(),
+                                    typename StatusHandler::base_type* status_out = nullptr) noexcept {
+        static_assert(detail::fixed_step_scheme<ProbingScheme>::value, "refill kernels need a fixed-step scheme");
+        if (!base_table_.is_initialized() || num_in == 0 || num_in > buf.capacity) return;
+        combine(keys_in, num_in, buf, stream);
+        detail::launch_insert_refill<add_view_type, status_handlers::ReturnStatus,
+                                     detail::refill_insert_keys<unsigned(cg_size())>(), true>(
+            buf.keys, buf.counts, num_in, add_view(), probing_length, buf.status, stream, buf.num);
+        finish_combined<StatusHandler>(buf, 0, num_in, num_in, stream, status_out);
+    }
+
+private:
+    // the distinct keys and counts of keys_in[0 .. n) into buf (entry count
+    // in *buf.num, cleared first by a fill kernel)
+    WC_H void combine(const key_type* keys_in, index_type n, const CombineBuffers& buf, hipStream_t stream) noexcept {
+        storage::kernels::fill<index_type><<<1, 64, 0, stream>>>(buf.num, 1, index_type(0));
+        const index_type tiles = SDIV(n, index_type(detail::kCombineTile));
+        const index_type limit = index_type(detail::num_compute_units()) * 4;
+        kernels::count_combine<key_type, value_type, EmptyKey, TombstoneKey>
+            <<<unsigned(tiles < limit ? tiles : limit), 256, 0, stream>>>(keys_in, n, buf.keys, buf.counts, buf.rep,
+                                                                          buf.pos, buf.num);
+    }
+
+    // table-status join of duplicates and the per-occurrence statuses of
+    // occurrences [first, first + n) from the entries' statuses
+    // (max_entries bounds the entry count read on the device)
+    template<class StatusHandler>
+    WC_H void finish_combined(const CombineBuffers& buf, index_type first, index_type n, index_type max_entries,
+                              hipStream_t stream, typename StatusHandler::base_type* status_out) noexcept {
+        kernels::count_join_duplicates<value_type>
+            <<<detail::grid_for(max_entries, 1, 4), block_size(), 0, stream>>>(buf.counts, buf.status, buf.num,
+                                                                              base_table_.status_);
+        if constexpr (!std::is_same<StatusHandler, status_handlers::ReturnNothing>::value)
+            if (status_out != nullptr)
+                kernels::count_expand<StatusHandler><<<detail::grid_for(n, 1, 8), block_size(), 0, stream>>>(
+                    buf.pos, buf.rep, buf.status, first, n, status_out);
+    }
+
+public:
     // counting_hash_table.cuh:236-249
     template<class StatusHandler = defaults::status_handler_t>
     WC_H void retrieve(const key_type* keys_in, index_type num_in, value_type* values_out, hipStream_t stream = 0,
@@ -479,39 +571,24 @@ public:
     WC_HD add_view_type add_view() const noexcept { return add_view_type{base_table_.view(), max_count_}; }
 
 private:
-    // device buffers of insert_combined, grown on demand
+    // device buffer of insert_combined, grown on demand
     struct Scratch {
-        key_type* keys = nullptr;
-        value_type* counts = nullptr;
-        std::uint32_t* rep = nullptr;
-        std::uint32_t* pos = nullptr;
-        Status* status = nullptr;
-        index_type* num = nullptr;
+        void* base = nullptr;
         index_type capacity = 0;
         WC_H bool reserve(index_type n) noexcept {
             if (n <= capacity) return true;
             release();
-            bool ok = hipMalloc(reinterpret_cast<void**>(&keys), sizeof(key_type) * n) == hipSuccess &&
-                      hipMalloc(reinterpret_cast<void**>(&counts), sizeof(value_type) * n) == hipSuccess &&
-                      hipMalloc(reinterpret_cast<void**>(&rep), sizeof(std::uint32_t) * n) == hipSuccess &&
-                      hipMalloc(reinterpret_cast<void**>(&pos), sizeof(std::uint32_t) * n) == hipSuccess &&
-                      hipMalloc(reinterpret_cast<void**>(&status), sizeof(Status) * n) == hipSuccess &&
-                      hipMalloc(reinterpret_cast<void**>(&num), sizeof(index_type)) == hipSuccess;
-            if (!ok) {
-                release();
+            if (hipMalloc(&base, CombineBuffers::bytes_for(n)) != hipSuccess) {
+                base = nullptr;
                 return false;
             }
             capacity = n;
             return true;
         }
+        WC_H CombineBuffers buffers() const noexcept { return CombineBuffers::carve(base, capacity); }
         WC_H void release() noexcept {
-            for (void* p : {(void*)keys, (void*)counts, (void*)rep, (void*)pos, (void*)status, (void*)num})
-                if (p) (void)hipFree(p);
-            keys = nullptr;
-            counts = nullptr;
-            rep = pos = nullptr;
-            status = nullptr;
-            num = nullptr;
+            if (base) (void)hipFree(base);
+            base = nullptr;
             capacity = 0;
         }
     };

@@ -32,7 +32,11 @@
 namespace warpcore {
 namespace detail {
 
-// Slots per group of the refill kernels.
+// Slots per group of the refill kernels (pipelined lookups: their own
+// count, since every slot's window stays in registers for a whole pass).
+#ifndef WARPCORE_REFILL_PIPE_KEYS
+#define WARPCORE_REFILL_PIPE_KEYS 8
+#endif
 #ifndef WARPCORE_REFILL_KEYS
 #define WARPCORE_REFILL_KEYS 8
 #endif
@@ -74,7 +78,7 @@ struct alignas(16) RefillEntry {
 // core.max_values values were met, with none, or key_not_found (joined)
 // if it has no value, and core.finish(i, count, room, status, join,
 // *tally) sees the key's value count.
-template<class Core, class StatusHandler, int U, class W>
+template<class Core, class StatusHandler, int U, class W, bool Pipe = false>
 WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys_in, index_t first, index_t end,
                                 typename Core::out_type* __restrict__ values_out, const Core& core, index_t budget,
                                 typename StatusHandler::base_type* __restrict__ status_out,
@@ -120,6 +124,7 @@ WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys
     W w[U], step[U];
     std::uint32_t pos[U], rnd[U];
     std::uint64_t live[U];
+    Slot s[U];   // the window each slot read (pipelined: in flight from the previous pass)
     index_t count[multi ? U : 1];   // multi-value: values of the key met so far
     index_t base[multi ? U : 1], room[multi ? U : 1];   // multi-value: the key's output range
 #pragma unroll
@@ -135,6 +140,7 @@ WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys
         pos[u] = 0;
         rnd[u] = 0;
         live[u] = 0;
+        s[u] = Slot{};
     }
     index_t next = 0;          // keys of the range staged so far (group-uniform)
     std::uint64_t avail = 0;   // stage entries holding a key not yet taken (group-uniform)
@@ -143,6 +149,11 @@ WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys
     // never waits for memory
     K ahead = K{};
     if (len > 0) ahead = keys_in[first + (rank < len ? rank : 0)];
+    if constexpr (Pipe) {   // every slot has a load in flight (window 0) from the start, issued
+                            // after `ahead`, so the first staging need not wait for the slots
+#pragma unroll
+        for (int u = 0; u < U; ++u) s[u] = load_slot(lane_slots);
+    }
 
     while (true) {
         // stage the next CG keys of the range once the stage is used up
@@ -150,7 +161,6 @@ WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys
             const bool in = next + rank < len;
             const index_t i = first + next + (in ? rank : 0);
             const K k = ahead;
-            if (next + CG < len) ahead = keys_in[first + next + CG + (next + CG + rank < len ? rank : 0)];
             const bool valid = in && core.is_valid_key(k);
             if (valid) {
                 Cursor c;
@@ -167,16 +177,21 @@ WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys
             }
             const std::uint64_t vb = ballot(valid);
             avail = CG == 64 ? vb : ((vb >> shift) & full);
+            {   // the next stage's keys, loaded once this stage's are used (the
+                // load then writes `ahead` in place) and unconditionally (the
+                // last key again past the end): a load that had to be copied
+                // into `ahead` would be waited for at once
+                const index_t nn = next + CG + rank;
+                ahead = keys_in[first + (nn < len ? nn : len - 1)];
+            }
             stage_pos = std::uint32_t(next);
             next += CG;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        // refill free slots from the stage, lowest entry first
-        std::uint64_t any_live = 0;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
+        // refill slot u from the stage, lowest entry first
+        auto take_slot = [&](int u) {
             const bool take = !mine(live[u]) && avail != 0;
             if (take) {
                 const unsigned j = unsigned(ffs64(avail) - 1);
@@ -193,21 +208,14 @@ WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys
                 }
             }
             live[u] |= ballot(take);
-            any_live |= live[u];
-        }
-        if (any_live == 0 && ballot(avail != 0 || next < len) == 0) break;
-        // staging reads and LDS reads of this round are done before the
-        // stage can be overwritten at the top of the next round
-        __builtin_amdgcn_wave_barrier();
-
-        Slot s[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {   // a free slot reads window 0: one line every free slot shares
+        };
+        // a free slot reads window 0: one line every free slot shares
+        auto issue_slot = [&](int u) {
             const W at = mine(live[u]) ? w[u] : W(0);
             s[u] = load_slot(lane_slots + index_t(at) * CG);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
+        };
+        // slot u's window has arrived: finish or advance its key
+        auto process_slot = [&](int u) {
             const K tk = Core::key_of(s[u]);
             const std::uint64_t hit = ballot(tk == key[u]) & live[u];
             if constexpr (multi) {
@@ -234,26 +242,58 @@ WC_D void retrieve_refill_range(const typename Core::key_type* __restrict__ keys
                 w[u] = mine(cont) ? (nw >= p ? W(nw - p) : nw) : w[u];
                 rnd[u] += 1;
                 live[u] = cont;
-                continue;
+            } else {
+                const std::uint64_t stop = spread(hit | (ballot(Core::is_empty_key(tk)) & live[u]));
+                const std::uint64_t ghit = spread(hit);
+                const std::uint64_t out_of_windows = live[u] & ~stop & ballot(rnd[u] + 1 >= budget);
+                const std::uint64_t missing = stop & ~ghit;
+                if (mine(hit)) out[pos[u]] = core.found(Core::value_of(s[u]));
+                missed |= missing;
+                exhausted |= out_of_windows;
+                if (rank == 0 && mine(stop | out_of_windows)) {
+                    const Status st = mine(ghit) ? Status::none()
+                                                 : (mine(missing) ? miss_status : Status::probing_length_exceeded());
+                    StatusHandler::handle(st, status_out, first + pos[u]);
+                    if (mine(missing) && !miss_status.has_any()) out[pos[u]] = miss_value;
+                }
+                const std::uint64_t cont = live[u] & ~stop & ~out_of_windows;
+                const W nw = w[u] + step[u];
+                w[u] = mine(cont) ? (nw >= p ? W(nw - p) : nw) : w[u];
+                rnd[u] += 1;
+                live[u] = cont;
             }
-            const std::uint64_t stop = spread(hit | (ballot(Core::is_empty_key(tk)) & live[u]));
-            const std::uint64_t ghit = spread(hit);
-            const std::uint64_t out_of_windows = live[u] & ~stop & ballot(rnd[u] + 1 >= budget);
-            const std::uint64_t missing = stop & ~ghit;
-            if (mine(hit)) out[pos[u]] = core.found(Core::value_of(s[u]));
-            missed |= missing;
-            exhausted |= out_of_windows;
-            if (rank == 0 && mine(stop | out_of_windows)) {
-                const Status st = mine(ghit) ? Status::none()
-                                             : (mine(missing) ? miss_status : Status::probing_length_exceeded());
-                StatusHandler::handle(st, status_out, first + pos[u]);
-                if (mine(missing) && !miss_status.has_any()) out[pos[u]] = miss_value;
+        };
+        std::uint64_t any_live = 0;
+        if constexpr (!Pipe) {
+            // one round: refill, every slot's load, then every slot's result
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                take_slot(u);
+                any_live |= live[u];
             }
-            const std::uint64_t cont = live[u] & ~stop & ~out_of_windows;
-            const W nw = w[u] + step[u];
-            w[u] = mine(cont) ? (nw >= p ? W(nw - p) : nw) : w[u];
-            rnd[u] += 1;
-            live[u] = cont;
+            if (any_live == 0 && ballot(avail != 0 || next < len) == 0) break;
+            // staging reads and LDS reads of this round are done before the
+            // stage can be overwritten at the top of the next round
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int u = 0; u < U; ++u) issue_slot(u);
+#pragma unroll
+            for (int u = 0; u < U; ++u) process_slot(u);
+        } else {
+            // pipelined: slot u's window (issued one pass earlier) is
+            // consumed and slot u's next load issued before slot u + 1 is
+            // looked at, so a pass waits only for the oldest load in flight
+            // (loads return in issue order) while the other slots' loads
+            // stay outstanding
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (live[u] != 0) process_slot(u);
+                take_slot(u);
+                any_live |= live[u];
+                issue_slot(u);
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (any_live == 0 && ballot(avail != 0 || next < len) == 0) break;
         }
     }
     if (mine(missed)) join(miss_status);
@@ -284,8 +324,10 @@ struct alignas(16) RefillInsertEntry {
 // when a CAS claims a slot (none if no value of the key was met, else
 // duplicate_key; not joined) or when the count reaches core.max_values
 // (duplicate_key + max_values_for_key_reached, joined). A claim with no
-// earlier value adds one to *new_keys.
-template<class Core, class StatusHandler, int U, class W, bool HasValues>
+// earlier value adds one to *new_keys. Single-value cores add one to
+// *new_keys per claimed slot (the keys reported `none`), in the group's
+// rank-0 lane.
+template<class Core, class StatusHandler, int U, class W, bool HasValues, bool Pipe = false>
 WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_in,
                               const typename Core::value_type* __restrict__ values_in, index_t first, index_t end,
                               const Core& core, index_t budget,
@@ -348,6 +390,16 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
         live[u] = 0;
         claim[u] = 0;
     }
+    // requests of the slots (pipelined: in flight from the previous pass)
+    Probe fresh[U];
+    typename Core::cas_word old[U];
+    bool lead[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        fresh[u] = Probe{};
+        old[u] = typename Core::cas_word{};
+        lead[u] = false;
+    }
     std::uint64_t dups = 0, exhausted = 0;   // groups that saw a duplicate (multi-value: reached the
                                              // maximum) / ran out of windows
     std::uint64_t exhausted_dup = 0;         // multi-value: ran out of windows after values of the key
@@ -362,6 +414,11 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
         ahead = keys_in[i0];
         if constexpr (HasValues) ahead_value = values_in[i0];
     }
+    if constexpr (Pipe) {   // every slot has a load in flight (window 0) from the start, issued
+                            // after `ahead`, so the first staging need not wait for the slots
+#pragma unroll
+        for (int u = 0; u < U; ++u) fresh[u] = Core::load_insert_probe(slots + rank);
+    }
 
     while (true) {
         if (avail == 0 && next < len) {
@@ -369,11 +426,6 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
             const index_t i = first + next + (in ? rank : 0);
             const K k = ahead;
             const V kv = ahead_value;
-            if (next + CG < len) {
-                const index_t i2 = first + next + CG + (next + CG + rank < len ? rank : 0);
-                ahead = keys_in[i2];
-                if constexpr (HasValues) ahead_value = values_in[i2];
-            }
             const bool valid = in && core.is_valid_key(k);
             if (valid) {
                 Cursor c;
@@ -390,15 +442,20 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
                 StatusHandler::handle(Status::invalid_key(), status_out, i);
             }
             avail = group_bits(ballot(valid));
+            {   // the next stage's pairs, loaded in place once this stage's are
+                // used, unconditionally (see retrieve_refill_range)
+                const index_t nn = next + CG + rank;
+                const index_t i2 = first + (nn < len ? nn : len - 1);
+                ahead = keys_in[i2];
+                if constexpr (HasValues) ahead_value = values_in[i2];
+            }
             stage_pos = std::uint32_t(next);
             next += CG;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        std::uint64_t any_live = 0;
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
+        auto take_slot = [&](int u) {
             const bool take = !mine(live[u]) && avail != 0;
             if (take) {
                 const unsigned j = unsigned(ffs64(avail) - 1);
@@ -415,22 +472,12 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
             const std::uint64_t t = ballot(take);
             live[u] |= t;
             claim[u] &= ~t;
-            any_live |= live[u];
-        }
-        if (any_live == 0 && ballot(avail != 0 || next < len) == 0) break;
-        __builtin_amdgcn_wave_barrier();
-
-        // issue: window loads of loading slots, claiming CASes
-        Probe fresh[U];
-        typename Core::cas_word old[U];
-        bool lead[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {   // a slot not loading reads window 0: one line it shares
+        };
+        // issue: the window load of a loading slot (a slot not loading reads
+        // window 0: one line it shares), the CAS of a claiming slot's leader
+        auto issue_slot = [&](int u) {
             const W at = mine(live[u] & ~claim[u]) ? w[u] : W(0);
             fresh[u] = Core::load_insert_probe(slots + index_t(at) * CG + rank);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
             lead[u] = mine(claim[u]) && rank == unsigned(ffs64(emp[u]) - 1);
             old[u] = typename Core::cas_word{};
             if (lead[u]) {
@@ -438,9 +485,8 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
                 if constexpr (keep_seen) expect = seen[u];
                 old[u] = core.cas_issue(slots + index_t(w[u]) * CG + rank, expect, key[u], val[u]);
             }
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
+        };
+        auto process_slot = [&](int u) {
             const std::uint64_t loading = live[u] & ~claim[u];
             const std::uint64_t claiming = live[u] & claim[u];
             // loading slots: a hit is a duplicate (multi-value: counted; the
@@ -503,6 +549,7 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
                     core.finish_insert(index_t(w[u]) * CG + at_lane, val[u], mine(won) && Core::publishes_value, st,
                                        join);
                     StatusHandler::handle(st, status_out, first + pos[u]);
+                    if (mine(won) && new_keys != nullptr) *new_keys += 1;
                 }
             }
             // slots moving to the next window, or out of windows
@@ -524,6 +571,34 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
             }
             claim[u] = (claim[u] | gemp) & ~(done | adv);
             live[u] &= ~(done | last);
+        };
+        std::uint64_t any_live = 0;
+        if constexpr (!Pipe) {
+            // one round: refill, every slot's load or CAS, then every result
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                take_slot(u);
+                any_live |= live[u];
+            }
+            if (any_live == 0 && ballot(avail != 0 || next < len) == 0) break;
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int u = 0; u < U; ++u) issue_slot(u);
+#pragma unroll
+            for (int u = 0; u < U; ++u) process_slot(u);
+        } else {
+            // pipelined: slot u's load or CAS (issued one pass earlier) is
+            // consumed and its next one issued before slot u + 1 is looked
+            // at, so a pass waits only for the oldest request in flight
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (live[u] != 0) process_slot(u);
+                take_slot(u);
+                any_live |= live[u];
+                issue_slot(u);
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (any_live == 0 && ballot(avail != 0 || next < len) == 0) break;
         }
     }
     if constexpr (multi) {
@@ -562,13 +637,12 @@ namespace kernels {
 #endif
 
 // Bulk lookups: group g of the grid takes keys [g * per, (g + 1) * per).
-template<class Core, class StatusHandler, int U, class W>
-__global__ __launch_bounds__(256) WC_REFILL_OCCUPANCY void retrieve_refill(const typename Core::key_type* __restrict__ keys_in,
-                                                       index_t num_in, typename Core::out_type* __restrict__ values_out,
-                                                       Core core, index_t probing_length,
-                                                       typename StatusHandler::base_type* __restrict__ status_out) {
+template<class Core, class StatusHandler, int U, class W, bool Pipe>
+WC_D void retrieve_refill_body(const typename Core::key_type* __restrict__ keys_in, index_t num_in,
+                               typename Core::out_type* __restrict__ values_out, const Core& core,
+                               index_t probing_length, typename StatusHandler::base_type* __restrict__ status_out,
+                               detail::RefillEntry<typename Core::key_type, W>* stage) {
     constexpr unsigned CG = unsigned(Core::cg_size());
-    __shared__ detail::RefillEntry<typename Core::key_type, W> stage[256];
     detail::DeferredJoin join;
     const index_t gid = (index_t(blockIdx.x) * blockDim.x + threadIdx.x) / CG;
     const index_t groups = (index_t(gridDim.x) * blockDim.x) / CG;
@@ -577,21 +651,60 @@ __global__ __launch_bounds__(256) WC_REFILL_OCCUPANCY void retrieve_refill(const
     const index_t end = first + per < num_in ? first + per : num_in;
     const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, core.geometry().windows);
     index_t tally = 0;
-    detail::retrieve_refill_range<Core, StatusHandler, U, W>(keys_in, first, end, values_out, core, budget,
-                                                             status_out, stage + (threadIdx.x & ~(CG - 1)), join,
-                                                             &tally);
+    detail::retrieve_refill_range<Core, StatusHandler, U, W, Pipe>(keys_in, first, end, values_out, core, budget,
+                                                                   status_out, stage + (threadIdx.x & ~(CG - 1)),
+                                                                   join, &tally);
     join.flush(core.status_word());
     if constexpr (detail::is_multi_value_core<Core>::value) core.add_tally(tally);
 }
 
-template<class Core, class StatusHandler, int U, class W, bool HasValues>
+template<class Core, class StatusHandler, int U, class W>
+__global__ __launch_bounds__(256) WC_REFILL_OCCUPANCY void retrieve_refill(const typename Core::key_type* __restrict__ keys_in,
+                                                       index_t num_in, typename Core::out_type* __restrict__ values_out,
+                                                       Core core, index_t probing_length,
+                                                       typename StatusHandler::base_type* __restrict__ status_out) {
+    __shared__ detail::RefillEntry<typename Core::key_type, W> stage[256];
+    retrieve_refill_body<Core, StatusHandler, U, W, false>(keys_in, num_in, values_out, core, probing_length,
+                                                           status_out, stage);
+}
+
+// Wavefronts per SIMD the pipelined refill lookup kernel is compiled for
+// (0: the compiler's choice); its loads in flight hold their registers
+// across a whole pass.
+#ifndef WARPCORE_REFILL_PIPE_WAVES
+#define WARPCORE_REFILL_PIPE_WAVES 0
+#endif
+#if WARPCORE_REFILL_PIPE_WAVES > 0
+#define WC_REFILL_PIPE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WARPCORE_REFILL_PIPE_WAVES)))
+#else
+#define WC_REFILL_PIPE_OCCUPANCY
+#endif
+template<class Core, class StatusHandler, int U, class W>
+__global__ __launch_bounds__(256) WC_REFILL_PIPE_OCCUPANCY void retrieve_refill_pipe(
+    const typename Core::key_type* __restrict__ keys_in, index_t num_in,
+    typename Core::out_type* __restrict__ values_out, Core core, index_t probing_length,
+    typename StatusHandler::base_type* __restrict__ status_out) {
+    __shared__ detail::RefillEntry<typename Core::key_type, W> stage[256];
+    retrieve_refill_body<Core, StatusHandler, U, W, true>(keys_in, num_in, values_out, core, probing_length,
+                                                          status_out, stage);
+}
+
+// Bulk inserts. num_dev (nullable): the key count is min(num_in, *num_dev),
+// read on the device, so a caller whose count is produced by an earlier
+// kernel on the stream needs no host round trip (num_in bounds it).
+// claims_out (nullable, single-value cores): the number of slots the call
+// claimed is added to *claims_out (one atomic per wavefront).
+template<class Core, class StatusHandler, int U, class W, bool HasValues, bool Pipe>
 __global__ __launch_bounds__(256) WC_REFILL_INSERT_OCCUPANCY void insert_refill(const typename Core::key_type* __restrict__ keys_in,
                                                      const typename Core::value_type* __restrict__ values_in,
                                                      index_t num_in, Core core, index_t probing_length,
-                                                     typename StatusHandler::base_type* __restrict__ status_out) {
+                                                     typename StatusHandler::base_type* __restrict__ status_out,
+                                                     const index_t* __restrict__ num_dev,
+                                                     index_t* __restrict__ claims_out) {
     constexpr unsigned CG = unsigned(Core::cg_size());
     __shared__ detail::RefillInsertEntry<typename Core::key_type, W, typename Core::value_type> stage[256];
     detail::DeferredJoin join;
+    if (num_dev != nullptr && *num_dev < num_in) num_in = *num_dev;
     const index_t gid = (index_t(blockIdx.x) * blockDim.x + threadIdx.x) / CG;
     const index_t groups = (index_t(gridDim.x) * blockDim.x) / CG;
     const index_t per = SDIV(num_in, groups);
@@ -599,11 +712,15 @@ __global__ __launch_bounds__(256) WC_REFILL_INSERT_OCCUPANCY void insert_refill(
     const index_t end = first + per < num_in ? first + per : num_in;
     const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, core.geometry().windows);
     index_t new_keys = 0;
-    detail::insert_refill_range<Core, StatusHandler, U, W, HasValues>(keys_in, values_in, first, end, core, budget,
-                                                                      status_out, stage + (threadIdx.x & ~(CG - 1)),
-                                                                      join, &new_keys);
+    detail::insert_refill_range<Core, StatusHandler, U, W, HasValues, Pipe>(
+        keys_in, values_in, first, end, core, budget, status_out, stage + (threadIdx.x & ~(CG - 1)), join, &new_keys);
     join.flush(core.status_word());
-    if constexpr (detail::is_multi_value_core<Core>::value) core.add_keys(new_keys);
+    if constexpr (detail::is_multi_value_core<Core>::value) {
+        core.add_keys(new_keys);
+    } else if (claims_out != nullptr) {
+        const index_t all = tile<64>::reduce_add(new_keys);
+        if (lane_id() == 0 && all != 0) detail::atomic_add(claims_out, all);
+    }
 }
 
 }  // namespace kernels
@@ -626,16 +743,49 @@ WC_H unsigned refill_grid() noexcept {
     return grid;
 }
 
+template<class Core, class StatusHandler, int U, bool HasValues, bool Pipe>
+WC_H void launch_insert_refill_as(const typename Core::key_type* keys_in, const typename Core::value_type* values_in,
+                                  index_t num_in, const Core& core, index_t probing_length,
+                                  typename StatusHandler::base_type* status_out, hipStream_t stream,
+                                  const index_t* num_dev, index_t* claims_out) noexcept {
+    if (core.geometry().windows < (index_t(1) << 31)) {
+        constexpr auto k = kernels::insert_refill<Core, StatusHandler, U, std::uint32_t, HasValues, Pipe>;
+        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out,
+                                                         num_dev, claims_out);
+    } else {
+        constexpr auto k = kernels::insert_refill<Core, StatusHandler, U, index_t, HasValues, Pipe>;
+        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out,
+                                                         num_dev, claims_out);
+    }
+}
+
 template<class Core, class StatusHandler, int U, bool HasValues>
 WC_H void launch_insert_refill(const typename Core::key_type* keys_in, const typename Core::value_type* values_in,
                                index_t num_in, const Core& core, index_t probing_length,
-                               typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
+                               typename StatusHandler::base_type* status_out, hipStream_t stream,
+                               const index_t* num_dev = nullptr, index_t* claims_out = nullptr) noexcept {
+    if (pipelined_inserts())
+        launch_insert_refill_as<Core, StatusHandler, U, HasValues, true>(keys_in, values_in, num_in, core,
+                                                                         probing_length, status_out, stream, num_dev,
+                                                                         claims_out);
+    else
+        launch_insert_refill_as<Core, StatusHandler, U, HasValues, false>(keys_in, values_in, num_in, core,
+                                                                          probing_length, status_out, stream, num_dev,
+                                                                          claims_out);
+}
+
+template<class Core, class StatusHandler, int U, bool Pipe>
+WC_H void launch_retrieve_refill_as(const typename Core::key_type* keys_in, index_t num_in,
+                                    typename Core::out_type* values_out, const Core& core, index_t probing_length,
+                                    typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
     if (core.geometry().windows < (index_t(1) << 31)) {
-        constexpr auto k = kernels::insert_refill<Core, StatusHandler, U, std::uint32_t, HasValues>;
-        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out);
+        constexpr auto k = Pipe ? kernels::retrieve_refill_pipe<Core, StatusHandler, U, std::uint32_t>
+                                : kernels::retrieve_refill<Core, StatusHandler, U, std::uint32_t>;
+        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
     } else {
-        constexpr auto k = kernels::insert_refill<Core, StatusHandler, U, index_t, HasValues>;
-        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out);
+        constexpr auto k = Pipe ? kernels::retrieve_refill_pipe<Core, StatusHandler, U, index_t>
+                                : kernels::retrieve_refill<Core, StatusHandler, U, index_t>;
+        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
     }
 }
 
@@ -643,13 +793,13 @@ template<class Core, class StatusHandler, int U>
 WC_H void launch_retrieve_refill(const typename Core::key_type* keys_in, index_t num_in,
                                  typename Core::out_type* values_out, const Core& core, index_t probing_length,
                                  typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
-    if (core.geometry().windows < (index_t(1) << 31)) {
-        constexpr auto k = kernels::retrieve_refill<Core, StatusHandler, U, std::uint32_t>;
-        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
-    } else {
-        constexpr auto k = kernels::retrieve_refill<Core, StatusHandler, U, index_t>;
-        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
-    }
+    constexpr int UP = U == WARPCORE_REFILL_KEYS ? WARPCORE_REFILL_PIPE_KEYS : U;
+    if (pipelined_lookups())
+        launch_retrieve_refill_as<Core, StatusHandler, UP, true>(keys_in, num_in, values_out, core, probing_length,
+                                                                 status_out, stream);
+    else
+        launch_retrieve_refill_as<Core, StatusHandler, U, false>(keys_in, num_in, values_out, core, probing_length,
+                                                                 status_out, stream);
 }
 
 }  // namespace detail

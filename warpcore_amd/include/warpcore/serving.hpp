@@ -26,11 +26,26 @@
 // steps.
 //
 // Admission against the HBM budget: every table has a slot budget of
-// max_load * capacity. An insert of n keys reserves n slots of its table
-// (a key may be a duplicate, so this is an upper bound); slots are never
-// given back, because erase leaves tombstones that keep their slots, so an
-// insert that does not fit its table's remaining budget is refused (status
+// max_load * capacity, counted in claimed slots (stored keys and the
+// tombstones erase leaves, which inserts never reuse). An insert of n keys
+// reserves n slots of its table when it is admitted (a key may already be
+// stored, so this is an upper bound); the slots it claims (counted by the
+// insert kernels) turn the reservation into claimed slots, and what is left
+// of it is given back when its last step has run. An insert that does not
+// fit waits until the steps in flight have run; if it still does not fit,
+// the table is compacted when it holds tombstones (its live pairs are read
+// out, the table re-initialised and the pairs inserted again, which drops
+// the tombstones; slot placement changes, results do not: with the default
+// probing length they depend only on the stored pairs while the table has
+// an empty slot, which the budget keeps); if it still does not fit, its keys
+// are looked up, and the ones not stored (an upper bound of the slots it can
+// claim, exact up to repeats inside the request) are its reservation
+// instead of n; an insert that does not fit even then is refused (status
 // out_of_memory for every key).
+//
+// Failures: a step whose launch or completion event reports an error fails
+// every request it holds (wait() returns an error code; outputs and statuses
+// of such a request are undefined).
 //
 // Order: per table, requests run in arrival order. A step takes, per
 // table, either any run of consecutive lookups, or one insert or erase (or
@@ -57,6 +72,7 @@
 #include <memory>
 #include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 namespace warpcore {
@@ -81,13 +97,15 @@ struct RequestDesc {
 namespace kernels {
 
 // Step preparation (one workgroup): copies the step's live descriptors from
-// the host-pinned staging buffer the scheduler wrote, counts each request's
-// work items and writes their exclusive scan (first item of each request).
+// the host-pinned staging buffer the scheduler wrote, clears their claim
+// counters, counts each request's work items and writes their exclusive
+// scan (first item of each request).
 template<class Desc>
 __global__ __launch_bounds__(256) void serve_prepare(const Desc* __restrict__ staged,
                                                      const std::uint32_t* __restrict__ staged_num, index_t max_reqs,
                                                      index_t item_keys, Desc* __restrict__ reqs,
-                                                     index_t* __restrict__ first, index_t* __restrict__ state) {
+                                                     index_t* __restrict__ first, index_t* __restrict__ state,
+                                                     index_t* __restrict__ claims) {
     const index_t num_raw = reinterpret_cast<const volatile std::uint32_t*>(staged_num)[0];
     const index_t num = num_raw < max_reqs ? num_raw : max_reqs;
     const index_t step_item_keys = reinterpret_cast<const volatile std::uint32_t*>(staged_num)[1];
@@ -99,6 +117,7 @@ __global__ __launch_bounds__(256) void serve_prepare(const Desc* __restrict__ st
         if (r < num) {
             const Desc d = staged[r];
             reqs[r] = d;
+            claims[r] = 0;
             k = SDIV(d.n, item_keys);
         }
         index_t total;
@@ -127,13 +146,15 @@ __global__ __launch_bounds__(256) void serve_prepare(const Desc* __restrict__ st
 // (items hold item_keys keys, so a static assignment balances and needs no
 // shared counter). An item's request is found by binary search in an LDS
 // copy of the requests' first items. Lookups and inserts use the refill
-// kernels' range code, erases the lockstep one.
+// kernels' range code, erases the lockstep one. claims[r]: slots the
+// inserts of request r claimed.
 template<class View, int UR, int UI, int UE>
 __global__ __launch_bounds__(256) WC_SERVE_OCCUPANCY void serve_step(const serving::RequestDesc<typename View::key_type,
                                                                              typename View::value_type>* __restrict__ reqs,
                                                   const index_t* __restrict__ first_item,
                                                   index_t* __restrict__ state, index_t max_reqs, index_t item_keys,
-                                                  const View* __restrict__ views, index_t probing_length) {
+                                                  const View* __restrict__ views, index_t probing_length,
+                                                  index_t* __restrict__ claims) {
     using K = typename View::key_type;
     using W = std::uint32_t;
     constexpr unsigned CG = unsigned(View::cg_size());
@@ -169,10 +190,12 @@ __global__ __launch_bounds__(256) WC_SERVE_OCCUPANCY void serve_step(const servi
             const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, view.geo.windows);
             detail::DeferredJoin join;
             auto* st = reinterpret_cast<Status*>(d.status_out);
-            if (d.op == std::uint32_t(serving::Op::insert))
+            if (d.op == std::uint32_t(serving::Op::insert)) {
+                index_t claimed = 0;
                 detail::insert_refill_range<View, H, UI, W, true>(d.keys, d.values, begin, end, view, budget, st,
-                                                                  my_refill_insert, join);
-            else if (d.op == std::uint32_t(serving::Op::retrieve))
+                                                                  my_refill_insert, join, &claimed);
+                if (rank == 0 && claimed != 0) detail::atomic_add(claims + r, claimed);
+            } else if (d.op == std::uint32_t(serving::Op::retrieve))
                 detail::retrieve_refill_range<View, H, UR, W>(d.keys, begin, end, d.values_out, view, budget, st,
                                                               my_refill, join);
             else
@@ -190,7 +213,15 @@ namespace serving {
 struct Stats {
     index_t requests = 0, steps = 0, keys = 0;
     index_t direct_steps = 0;   // steps run on a table's own bulk kernels
+    index_t compactions = 0;    // tables rebuilt to drop their tombstones
+    index_t failed = 0;         // requests that failed with their step
 };
+
+// wait() results
+constexpr int kServed = 1;
+constexpr int kRefused = 0;        // out of budget: every key reported out_of_memory
+constexpr int kFailed = -6;        // a step of the request failed to launch or to complete
+constexpr int kUnknownId = -7;     // never issued, or collected and dropped from the history
 
 // K/V: u32 or u64; CG: probing group (8 for 16-byte slots, 16 for 8-byte).
 template<class K, class V, index_t CG>
@@ -214,6 +245,9 @@ public:
         // its table's bulk kernels (0: every step runs the step kernel).
         index_t direct_keys = index_t(1) << 18;
         index_t direct_backlog = index_t(1) << 24;
+        // completed requests whose outcome stays available to wait() /
+        // latency_us() / step_of() (older ones are dropped)
+        index_t history = index_t(1) << 20;
     };
 
     struct Request {
@@ -224,6 +258,10 @@ public:
         std::uint64_t step = 0;          // step that finished it (0 = not yet)
         std::atomic<int> state{0};       // 0 queued, 1 running, 2 done
         bool refused = false;
+        bool failed = false;
+        bool probed = false;    // its keys were looked up for admission
+        index_t need = 0;       // probed: keys of it not stored
+        index_t held = 0;       // reservation not yet claimed
     };
 
     Server(const index_t* table_capacities, index_t num_tables, Config cfg = Config{}) : cfg_(cfg) {
@@ -240,13 +278,16 @@ public:
             }
             budget_.push_back(index_t(double(tables_.back()->capacity()) * cfg_.max_load));
             reserved_.push_back(0);
+            compacted_.push_back(0);
         }
         std::vector<view_type> views;
         for (auto& t : tables_) views.push_back(t->view());
         ok_ = alloc(&views_d_, views.size()) && alloc(&first_, cfg_.max_requests) && alloc(&state_, 3);
         for (int b = 0; b < 2 && ok_; ++b) {
-            ok_ = alloc(&desc_d_[b], cfg_.max_requests) &&
+            ok_ = alloc(&desc_d_[b], cfg_.max_requests) && alloc(&claims_d_[b], cfg_.max_requests) &&
                   hipHostMalloc(reinterpret_cast<void**>(&desc_h_[b]), sizeof(desc_type) * cfg_.max_requests) ==
+                      hipSuccess &&
+                  hipHostMalloc(reinterpret_cast<void**>(&claims_h_[b]), sizeof(index_t) * cfg_.max_requests) ==
                       hipSuccess &&
                   hipHostMalloc(reinterpret_cast<void**>(&num_reqs_h_[b]), 4 * sizeof(std::uint32_t)) == hipSuccess;
         }
@@ -276,10 +317,12 @@ public:
         if (worker_.joinable()) worker_.join();
         if (stream_) (void)hipStreamSynchronize(stream_);
         for (auto& g : graphs_) if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        for (void* p : {(void*)views_d_, (void*)first_, (void*)state_, (void*)desc_d_[0], (void*)desc_d_[1]})
+        for (void* p : {(void*)views_d_, (void*)first_, (void*)state_, (void*)desc_d_[0], (void*)desc_d_[1],
+                        (void*)claims_d_[0], (void*)claims_d_[1]})
             if (p) (void)hipFree(p);
         for (int b = 0; b < 2; ++b) {
             if (desc_h_[b]) (void)hipHostFree(desc_h_[b]);
+            if (claims_h_[b]) (void)hipHostFree(claims_h_[b]);
             if (num_reqs_h_[b]) (void)hipHostFree(num_reqs_h_[b]);
         }
         if (stream_) (void)hipStreamDestroy(stream_);
@@ -307,7 +350,8 @@ public:
             std::lock_guard<std::mutex> g(mu_);
             id = next_id_++;
             r->id = id;
-            all_.push_back(r);
+            all_.emplace(id, r);
+            ++outstanding_;
             if (bad) {
                 finish_refused(*r);
             } else {
@@ -319,33 +363,47 @@ public:
         return id;
     }
 
-    // Block until request `id` is done; returns true unless it was refused.
-    bool wait(index_t id) {
+    // Block until request `id` is done: kServed, kRefused, kFailed, or
+    // kUnknownId for an id never issued or dropped from the history.
+    int wait(index_t id) {
         std::unique_lock<std::mutex> g(mu_);
-        done_cv_.wait(g, [&] { return all_[id]->state.load() == 2; });
-        return !all_[id]->refused;
+        auto r = find(id);
+        if (!r) return kUnknownId;
+        done_cv_.wait(g, [&] { return r->state.load() == 2; });
+        return r->failed ? kFailed : (r->refused ? kRefused : kServed);
     }
     void drain() {
         std::unique_lock<std::mutex> g(mu_);
-        done_cv_.wait(g, [&] {
-            for (auto& r : all_) if (r->state.load() != 2) return false;
-            return true;
-        });
+        done_cv_.wait(g, [&] { return outstanding_ == 0; });
     }
-    // latency of request id in microseconds (arrival to completion seen)
+    // latency of request id in microseconds (arrival to completion seen);
+    // negative for an unknown id or a request not done yet
     double latency_us(index_t id) {
         std::lock_guard<std::mutex> g(mu_);
-        const auto& r = *all_[id];
-        return std::chrono::duration<double, std::micro>(r.done - r.arrival).count();
+        auto r = find(id);
+        if (!r || r->state.load() != 2) return -1.0;
+        return std::chrono::duration<double, std::micro>(r->done - r->arrival).count();
     }
     std::uint64_t step_of(index_t id) {
         std::lock_guard<std::mutex> g(mu_);
-        return all_[id]->step;
+        auto r = find(id);
+        return r ? r->step : 0;
     }
     Stats stats() {
         std::lock_guard<std::mutex> g(mu_);
         return stats_;
     }
+    // slots of table t counted against its budget (claimed, plus the
+    // reservations of inserts still running)
+    index_t reserved(index_t t) {
+        std::lock_guard<std::mutex> g(mu_);
+        return t < reserved_.size() ? reserved_[t] : 0;
+    }
+    index_t budget(index_t t) const noexcept { return t < budget_.size() ? budget_[t] : 0; }
+    // Failure handling from the host: step number `step` (1, 2, ... in
+    // issue order) is not launched and reports a launch failure, so its
+    // requests fail (wait() returns kFailed) while later steps run.
+    void fail_step(std::uint64_t step) noexcept { fail_step_.store(step); }
 
 private:
     struct Chunk {
@@ -361,18 +419,36 @@ private:
         return hipMalloc(reinterpret_cast<void**>(p), sizeof(T) * (n ? n : 1)) == hipSuccess;
     }
 
+    // under mu_
+    std::shared_ptr<Request> find(index_t id) const {
+        const auto it = all_.find(id);
+        return it == all_.end() ? nullptr : it->second;
+    }
+
+    // under mu_: request r is done; keep its outcome for the last
+    // cfg_.history completions
+    void complete(Request& r) {
+        r.done = clock::now();
+        r.state.store(2);
+        --outstanding_;
+        history_.push_back(r.id);
+        while (history_.size() > cfg_.history) {
+            all_.erase(history_.front());
+            history_.pop_front();
+        }
+        done_cv_.notify_all();
+    }
+
     void finish_refused(Request& r) {
         // every key of a refused request reports out_of_memory; on a side
-        // stream, as the step stream may be capturing a graph
+        // stream, as the step stream may be busy
         r.refused = true;
         if (r.d.status_out && r.d.n)
             storage::kernels::fill<std::uint32_t>
                 <<<detail::grid_for(r.d.n, 1, 8), block_size(), 0, side_>>>(r.d.status_out, r.d.n,
                                                                             Status::out_of_memory().base());
-        (void)hipStreamSynchronize(side_);
-        r.done = clock::now();
-        r.state.store(2);
-        done_cv_.notify_all();
+        if (hipStreamSynchronize(side_) != hipSuccess) r.failed = true;
+        complete(r);
     }
 
     // Requests (chunks) of the next step, chosen under mu_: per table in
@@ -381,12 +457,22 @@ private:
     // chunk descriptors to `descs`. In latency mode (short backlog) a large
     // request is taken alone, whole, and `direct` is set: the step runs on
     // the table's own bulk kernels; so does a step that holds one whole
-    // request.
-    void pick(StepList& step, desc_type* descs, bool& direct) {
+    // request. An insert that does not fit waits while steps are in flight
+    // (`idle` false: their unused reservations come back when they retire),
+    // and no request after it is taken until they have; with none in flight
+    // it asks for its table to be compacted (`compact` = the table) unless
+    // that was done since the table's last erase, then for its keys to be
+    // looked up (`probe`: the ones not stored are its reservation), and is
+    // refused when it does not fit even then.
+    void pick(StepList& step, desc_type* descs, bool& direct, bool idle, index_t& compact,
+              std::shared_ptr<Request>& probe) {
         std::vector<int> mode(tables_.size(), -1);   // -1 untouched, 0 lookups, 1 mutation, 2 closed
         std::deque<std::shared_ptr<Request>> keep;
         index_t keys = 0;
         direct = false;
+        compact = ~index_t(0);
+        probe = nullptr;
+        bool hold = false;   // an insert waits for the device to go idle: take nothing more
         index_t backlog = 0;
         if (cfg_.direct_keys != 0)
             for (const auto& r : pending_) backlog += r->d.n - r->issued;
@@ -396,16 +482,32 @@ private:
             pending_.pop_front();
             const std::uint32_t t = r->d.table;
             const Op op = Op(r->d.op);
-            bool take = !direct && step.size() < cfg_.max_requests && keys < cfg_.max_step_keys && mode[t] != 2 &&
-                        mode[t] != 1 && (op == Op::retrieve || mode[t] == -1);
+            bool take = !direct && !hold && compact == ~index_t(0) && !probe && step.size() < cfg_.max_requests &&
+                        keys < cfg_.max_step_keys && mode[t] != 2 && mode[t] != 1 &&
+                        (op == Op::retrieve || mode[t] == -1);
             const bool large = latency && r->d.n - r->issued >= cfg_.direct_keys;
             if (take && large && !step.empty()) take = false;   // runs alone in a later step
-            if (take && op == Op::insert && r->issued == 0) {
-                if (reserved_[t] + r->d.n > budget_[t]) {
+            const index_t want = r->probed ? r->need : r->d.n;
+            if (take && op == Op::insert && r->issued == 0 && reserved_[t] + want > budget_[t]) {
+                if (!idle || !step.empty()) {
+                    // wait for the steps in flight (and this one) to retire;
+                    // no new step starts meanwhile, so the device goes idle
+                    take = false;
+                    hold = true;
+                } else if (!compacted_[t]) {
+                    take = false;
+                    compact = t;
+                } else if (!r->probed) {
+                    take = false;
+                    probe = r;
+                } else {
                     finish_refused(*r);
                     continue;
                 }
-                reserved_[t] += r->d.n;
+            }
+            if (take && op == Op::insert && r->issued == 0) {
+                reserved_[t] += want;
+                r->held = want;
             }
             if (!take) {
                 mode[t] = 2;   // later requests of this table wait behind it
@@ -440,6 +542,69 @@ private:
         waiting_.store(pending_.size(), std::memory_order_release);
     }
 
+    // Rebuilds table t without its tombstones (no step in flight): its live
+    // pairs are read out, the table is re-initialised and they are inserted
+    // again. Its claimed slots are then its live pairs. Returns false on a
+    // device error.
+    bool compact_table(index_t t) {
+        table_type& tab = *tables_[t];
+        (void)hipGetLastError();   // drop the not-ready of earlier event polls
+        const index_t live = tab.size(stream_);
+        bool ok = hipGetLastError() == hipSuccess;
+        if (ok && live < reserved_[t]) {
+            K* keys = nullptr;
+            V* vals = nullptr;
+            ok = alloc(&keys, live) && alloc(&vals, live);
+            if (ok) {
+                index_t got = 0;
+                tab.retrieve_all(keys, vals, got, stream_);
+                ok = got == live;
+                if (ok) {
+                    tab.init(stream_);
+                    tab.insert(keys, vals, got, stream_);
+                    ok = hipStreamSynchronize(stream_) == hipSuccess;
+                }
+            }
+            if (keys) (void)hipFree(keys);
+            if (vals) (void)hipFree(vals);
+            if (ok) {
+                std::lock_guard<std::mutex> g(mu_);
+                reserved_[t] = live;
+                stats_.compactions++;
+            }
+        }
+        std::lock_guard<std::mutex> g(mu_);
+        compacted_[t] = 1;
+        return ok;
+    }
+
+    // Admission probe (no step in flight): the keys of insert r not stored
+    // in its table, counted from a lookup of all of them. Returns false on a
+    // device error (r is then refused).
+    bool probe_request(Request& r) {
+        table_type& tab = *tables_[r.d.table];
+        const index_t n = r.d.n;
+        std::uint32_t* st = nullptr;
+        V* out = nullptr;
+        std::vector<std::uint32_t> host(n);
+        bool ok = alloc(&st, n) && alloc(&out, n);
+        if (ok) {
+            tab.template retrieve<status_handlers::ReturnStatus>(r.d.keys, n, out, stream_, defaults::probing_length(),
+                                                                reinterpret_cast<Status*>(st));
+            ok = hipMemcpyAsync(host.data(), st, sizeof(std::uint32_t) * n, hipMemcpyDeviceToHost, stream_) ==
+                     hipSuccess &&
+                 hipStreamSynchronize(stream_) == hipSuccess;
+        }
+        if (st) (void)hipFree(st);
+        if (out) (void)hipFree(out);
+        index_t missing = 0;
+        for (const std::uint32_t x : host) missing += x != Status::none().base();
+        std::lock_guard<std::mutex> g(mu_);
+        r.probed = true;
+        r.need = ok ? missing : r.d.n;
+        return ok;
+    }
+
     struct Graph {
         hipGraphExec_t exec = nullptr;
     };
@@ -447,12 +612,14 @@ private:
     // One step: preparation, work-item listing and the persistent kernel,
     // reading staging buffer b (fixed shape: max_requests descriptors, of
     // which the first *num_reqs_h_[b] are live).
-    void enqueue_step(int b, hipStream_t stream) {
+    hipError_t enqueue_step(int b, hipStream_t stream) {
+        (void)hipGetLastError();   // drop the not-ready of earlier event polls: report only these launches
         const index_t R = cfg_.max_requests;
         kernels::serve_prepare<desc_type><<<1, 256, 0, stream>>>(desc_h_[b], num_reqs_h_[b], R, cfg_.item_keys,
-                                                                 desc_d_[b], first_, state_);
+                                                                 desc_d_[b], first_, state_, claims_d_[b]);
         step_kernel()<<<step_grid_, block_size(), sizeof(index_t) * R, stream>>>(
-            desc_d_[b], first_, state_, R, cfg_.item_keys, views_d_, defaults::probing_length());
+            desc_d_[b], first_, state_, R, cfg_.item_keys, views_d_, defaults::probing_length(), claims_d_[b]);
+        return hipGetLastError();
     }
 
     WC_H static auto step_kernel() noexcept {
@@ -474,9 +641,10 @@ private:
         index_t* first = first_;
         const index_t* first_c = first_;
         index_t* state = state_;
+        index_t* claims = claims_d_[b];
         const view_type* views = views_d_;
-        void* prep_args[] = {&staged, &staged_num, &R, &item_keys, &reqs, &first, &state};
-        void* step_args[] = {&reqs_c, &first_c, &state, &R, &item_keys, &views, &probing_length};
+        void* prep_args[] = {&staged, &staged_num, &R, &item_keys, &reqs, &first, &state, &claims};
+        void* step_args[] = {&reqs_c, &first_c, &state, &R, &item_keys, &views, &probing_length, &claims};
         hipKernelNodeParams prep{};
         prep.func = reinterpret_cast<void*>(kernels::serve_prepare<desc_type>);
         prep.gridDim = dim3(1);
@@ -501,41 +669,48 @@ private:
     }
 
     // A step of one request on its table's own bulk kernels (the kernels of
-    // one plain call), on the step stream.
-    void launch_direct(const desc_type& d) {
+    // one plain call), on the step stream; an insert counts its claimed
+    // slots into claims_d_[b][0].
+    hipError_t launch_direct(const desc_type& d, int b) {
+        (void)hipGetLastError();   // drop the not-ready of earlier event polls: report only these launches
         table_type& t = *tables_[d.table];
         const index_t pl = defaults::probing_length();
         auto* st = reinterpret_cast<Status*>(d.status_out);
+        using H = status_handlers::ReturnStatus;
         switch (Op(d.op)) {
-        case Op::insert:
-            if (st) t.template insert<status_handlers::ReturnStatus>(d.keys, d.values, d.n, stream_, pl, st);
-            else t.insert(d.keys, d.values, d.n, stream_, pl);
+        case Op::insert: {
+            constexpr int U = detail::refill_insert_keys<unsigned(CG)>();
+            storage::kernels::fill<index_t><<<1, 64, 0, stream_>>>(claims_d_[b], 1, index_t(0));
+            if (st)
+                detail::launch_insert_refill<view_type, H, U, true>(d.keys, d.values, d.n, t.view(), pl, st, stream_,
+                                                                    nullptr, claims_d_[b]);
+            else
+                detail::launch_insert_refill<view_type, status_handlers::ReturnNothing, U, true>(
+                    d.keys, d.values, d.n, t.view(), pl, nullptr, stream_, nullptr, claims_d_[b]);
             break;
+        }
         case Op::retrieve:
-            if (st) t.template retrieve<status_handlers::ReturnStatus>(d.keys, d.n, d.values_out, stream_, pl, st);
+            if (st) t.template retrieve<H>(d.keys, d.n, d.values_out, stream_, pl, st);
             else t.retrieve(d.keys, d.n, d.values_out, stream_, pl);
             break;
         case Op::erase:
-            if (st) t.template erase<status_handlers::ReturnStatus>(d.keys, d.n, stream_, pl, st);
+            if (st) t.template erase<H>(d.keys, d.n, stream_, pl, st);
             else t.erase(d.keys, d.n, stream_, pl);
             break;
         }
+        return hipGetLastError();
     }
 
-    void launch(int b) {
+    hipError_t launch(int b) {
         // the step reads its request count and descriptors from the pinned
         // staging buffer b, written by this thread before the launch
         std::atomic_thread_fence(std::memory_order_release);
-        if (!cfg_.use_graph) {
-            enqueue_step(b, stream_);
-            return;
-        }
+        if (!cfg_.use_graph) return enqueue_step(b, stream_);
         if (!graphs_[b].exec && !build_graph(b)) {
             cfg_.use_graph = false;
-            enqueue_step(b, stream_);
-            return;
+            return enqueue_step(b, stream_);
         }
-        (void)hipGraphLaunch(graphs_[b].exec, stream_);
+        return hipGraphLaunch(graphs_[b].exec, stream_);
     }
 
     // Keys per work item of a step: the smallest power of two from 8 that
@@ -556,23 +731,46 @@ private:
         hipEvent_t ev;
         std::uint64_t step_no;
         StepList chunks;
+        int b;                 // staging buffer (its claim counters)
+        bool inserts;          // claims copied to claims_h_[b]
+        bool failed;           // launch or completion error
     };
 
+    // The oldest step is done (or failed): give back the slots its inserts
+    // did not claim, reopen compaction of tables it erased from, and
+    // complete the requests whose last chunk it ran.
     void retire(std::deque<InFlight>& inflight) {
-        const auto now = clock::now();
+        InFlight& f = inflight.front();
         {
             std::lock_guard<std::mutex> g(mu_);
-            for (auto& c : inflight.front().chunks) {
+            for (std::size_t i = 0; i < f.chunks.size(); ++i) {
+                auto& c = f.chunks[i];
+                const std::uint32_t t = c.r->d.table;
+                if (c.r->d.op == std::uint32_t(Op::insert)) {
+                    // claimed slots leave the reservation (and stay counted);
+                    // a failed step may have claimed any part of its chunk, so
+                    // a failed request keeps its whole reservation
+                    if (!f.failed) {
+                        const index_t claimed = claims_h_[f.b][i];
+                        c.r->held -= claimed < c.r->held ? claimed : c.r->held;
+                        if (c.last && !c.r->failed) {
+                            reserved_[t] -= c.r->held;
+                            c.r->held = 0;
+                        }
+                    }
+                } else if (c.r->d.op == std::uint32_t(Op::erase)) {
+                    compacted_[t] = 0;
+                }
+                if (f.failed) c.r->failed = true;
                 if (!c.last) continue;
-                c.r->done = now;
-                c.r->step = inflight.front().step_no;
-                c.r->state.store(2);
+                c.r->step = f.step_no;
                 stats_.requests++;
                 stats_.keys += c.r->d.n;
+                if (c.r->failed) stats_.failed++;
+                complete(*c.r);
             }
         }
-        done_cv_.notify_all();
-        (void)hipEventDestroy(inflight.front().ev);
+        (void)hipEventDestroy(f.ev);
         inflight.pop_front();
     }
 
@@ -583,36 +781,61 @@ private:
         while (true) {
             StepList step;
             bool direct = false;
+            index_t compact = ~index_t(0);
+            std::shared_ptr<Request> probe;
             {
                 std::unique_lock<std::mutex> g(mu_);
                 if (inflight.empty()) cv_.wait(g, [&] { return stop_ || !pending_.empty(); });
                 if (stop_ && pending_.empty() && inflight.empty()) break;
                 // at most two steps in flight: descriptors are double-buffered
-                if (inflight.size() < 2) pick(step, desc_h_[b], direct);
+                if (inflight.size() < 2) pick(step, desc_h_[b], direct, inflight.empty(), compact, probe);
                 for (auto& c : step) c.r->state.store(1);
                 if (!step.empty()) stats_.steps++;
                 if (direct) stats_.direct_steps++;
             }
+            if (compact != ~index_t(0)) {   // no step in flight: rebuild, then pick again
+                (void)compact_table(compact);
+                continue;
+            }
+            if (probe) {   // no step in flight: count the insert's keys not stored, then pick again
+                (void)probe_request(*probe);
+                continue;
+            }
             if (!step.empty()) {
                 ++step_no;
-                if (direct) {
-                    launch_direct(desc_h_[b][0]);
+                bool inserts = false;
+                for (const auto& c : step) inserts |= c.r->d.op == std::uint32_t(Op::insert);
+                hipError_t err;
+                if (step_no == fail_step_.load()) {
+                    err = hipErrorLaunchFailure;
+                } else if (direct) {
+                    err = launch_direct(desc_h_[b][0], b);
                 } else {
                     num_reqs_h_[b][0] = std::uint32_t(step.size());
                     num_reqs_h_[b][1] = std::uint32_t(step_item_keys(step));
-                    launch(b);
+                    err = launch(b);
                 }
+                if (err == hipSuccess && inserts)
+                    err = hipMemcpyAsync(claims_h_[b], claims_d_[b], sizeof(index_t) * step.size(),
+                                         hipMemcpyDeviceToHost, stream_);
                 hipEvent_t ev = nullptr;
-                (void)hipEventCreateWithFlags(&ev, hipEventDisableTiming);
-                (void)hipEventRecord(ev, stream_);
-                inflight.push_back(InFlight{ev, step_no, std::move(step)});
+                if (err == hipSuccess) err = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+                if (err == hipSuccess) err = hipEventRecord(ev, stream_);
+                if (err != hipSuccess && ev != nullptr) {
+                    (void)hipEventDestroy(ev);
+                    ev = nullptr;
+                }
+                inflight.push_back(InFlight{ev, step_no, std::move(step), b, inserts, err != hipSuccess});
                 b ^= 1;
             }
             // retire finished steps; with a free staging buffer, go back and
             // build the next step as soon as requests are pending (spinning:
             // the scheduler thread owns its core)
             while (!inflight.empty()) {
-                if (hipEventQuery(inflight.front().ev) == hipSuccess) {
+                InFlight& f = inflight.front();
+                const hipError_t q = f.ev ? hipEventQuery(f.ev) : hipSuccess;
+                if (q != hipErrorNotReady) {
+                    if (q != hipSuccess) f.failed = true;
                     retire(inflight);
                     continue;
                 }
@@ -621,7 +844,7 @@ private:
                     std::this_thread::yield();
                     continue;
                 }
-                (void)hipEventSynchronize(inflight.front().ev);
+                if (hipEventSynchronize(f.ev) != hipSuccess) f.failed = true;
                 retire(inflight);
             }
         }
@@ -631,9 +854,12 @@ private:
     bool ok_ = true;
     std::vector<std::unique_ptr<table_type>> tables_;
     std::vector<index_t> budget_, reserved_;
+    std::vector<char> compacted_;   // compacted since the table's last erase
     view_type* views_d_ = nullptr;
     desc_type* desc_d_[2] = {nullptr, nullptr};
     desc_type* desc_h_[2] = {nullptr, nullptr};
+    index_t* claims_d_[2] = {nullptr, nullptr};   // per request of a step: slots its inserts claimed
+    index_t* claims_h_[2] = {nullptr, nullptr};
     std::uint32_t* num_reqs_h_[2] = {nullptr, nullptr};
     index_t *first_ = nullptr, *state_ = nullptr;   // state: live requests, work items, keys per item
     unsigned step_grid_ = 0;
@@ -643,11 +869,14 @@ private:
     std::mutex mu_;
     std::condition_variable cv_, done_cv_;
     std::deque<std::shared_ptr<Request>> pending_;
-    std::vector<std::shared_ptr<Request>> all_;
+    std::unordered_map<index_t, std::shared_ptr<Request>> all_;   // issued, not dropped from the history
+    std::deque<index_t> history_;                                 // completed ids, oldest first
+    index_t outstanding_ = 0;                                     // issued, not completed
     index_t next_id_ = 0;
     bool stop_ = false;
     std::atomic<bool> stop_flag_{false};   // stop_, readable without mu_
     std::atomic<std::size_t> waiting_{0};  // pending_.size(), readable without mu_
+    std::atomic<std::uint64_t> fail_step_{0};
     Stats stats_;
     std::thread worker_;
 };

@@ -745,8 +745,17 @@ class Server:
             raise ValueError("request refused at submission")
         return rid
 
+    SERVED, REFUSED, FAILED, UNKNOWN_ID = 1, 0, -6, -7
+
     def wait(self, rid: int) -> bool:
-        return self._fn("wait")(self._h, rid) == 1
+        """True when served, False when refused (out of budget); raises when
+        a step of the request failed or the id is unknown."""
+        rc = self._fn("wait")(self._h, rid)
+        if rc == self.SERVED:
+            return True
+        if rc == self.REFUSED:
+            return False
+        raise RuntimeError(f"request {rid}: wait returned {rc}")
 
     def drain(self):
         _check(self._fn("drain")(self._h), "drain")
@@ -760,8 +769,26 @@ class Server:
     def stats(self):
         out = (ctypes.c_uint64 * 3)()
         _check(self._fn("stats")(self._h, out), "stats")
+        c = (ctypes.c_uint64 * 4)()
+        _check(self._fn("counters")(self._h, c), "counters")
         return {"requests": out[0], "steps": out[1], "keys": out[2],
-                "direct_steps": self._fn("direct_steps")(self._h)}
+                "direct_steps": c[0], "compactions": c[1], "failed": c[2]}
+
+    def table_reserved(self, table: int) -> int:
+        return self._fn("table_reserved")(self._h, table)
+
+    def table_budget(self, table: int) -> int:
+        return self._fn("table_budget")(self._h, table)
+
+    def fail_step(self, step: int):
+        """Failure handling from the host: step `step` (1, 2, ...) is not
+        launched and reports a launch failure."""
+        _check(self._fn("fail_step")(self._h, step), "fail_step")
+
+    def wait_code(self, rid: int) -> int:
+        """wait() as the C ABI reports it: 1 served, 0 refused, -6 failed,
+        -7 unknown id."""
+        return self._fn("wait")(self._h, rid)
 
     def table_size(self, table: int) -> int:
         return self._fn("table_size")(self._h, table)
