@@ -254,6 +254,27 @@ WC_API uint32_t wc_counting_u64_peek_status(void* table, void* stream);
 WC_API uint32_t wc_counting_u64_pop_status(void* table, void* stream);
 WC_API uint32_t wc_counting_u64_cg_size(void* table);
 
+/*
+ * wc_counting_u64_insert_noalloc -- no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the same results,
+ * counts, per-occurrence statuses (which occurrence of a new key reports
+ * `none` is unspecified, as in _insert) and table status as
+ * wc_counting_u64_insert, with the combining buffers in a fixed device array
+ * of the library (28 MiB per GPU, held until the process ends) instead of
+ * memory the table allocates, and no host synchronisation. It calls no
+ * hipMalloc/hipFree and no allocating library routine.
+ *   num_in at most wc_counting_u64_insert_noalloc_max() (2^20); larger
+ *   inputs return WC_ERR_TOO_LARGE (-4) and change nothing.
+ *   Returns WC_ERR_NO_SCRATCH (-5) when the array's device address cannot
+ *   be obtained, else 0 or the hipError_t of the launch, -1 for NULL.
+ *   Calls that overlap on one GPU (other streams, other host threads) share
+ *   the array and race; calls one after another on one stream do not.
+ */
+#define WC_ERR_TOO_LARGE (-4)
+#define WC_ERR_NO_SCRATCH (-5)
+WC_API int wc_counting_u64_insert_noalloc(void* table, const uint64_t* keys_in, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
+WC_API uint64_t wc_counting_u64_insert_noalloc_max(void);
 
 /* ------------------------------------------------------------------------
  * HashSet<K> (include/warpcore/hash_set.cuh)

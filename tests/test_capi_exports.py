@@ -44,3 +44,13 @@ def test_probes_library_exports_its_header():
     assert all(hasattr(lib, n) for n in protos)
     assert not any(n.startswith("wc_util_") for n in _capi.parse_header())
 
+
+def test_noalloc_entry_point_declared():
+    """The insert that allocates nothing is declared beside the others, with
+    its size limit exported."""
+    protos = _capi.parse_header()
+    ret, args = protos["wc_counting_u64_insert_noalloc"]
+    assert ret is ctypes.c_int and len(args) == 6
+    lib = ctypes.CDLL(str(_library_path()))
+    lib.wc_counting_u64_insert_noalloc_max.restype = ctypes.c_uint64
+    assert lib.wc_counting_u64_insert_noalloc_max() == 1 << 20

@@ -16,6 +16,8 @@ inline int last_error() noexcept { return int(hipGetLastError()); }
 constexpr int kBadHandle = -1;
 constexpr int kBadArgument = -2;
 constexpr int kNoMemory = -3;
+constexpr int kTooLarge = -4;    // input larger than a fixed-size buffer of the call
+constexpr int kNoScratch = -5;   // device address of a static scratch array unavailable
 
 template<class F>
 inline int guarded(F&& f) noexcept {

@@ -4,6 +4,8 @@
 
 #include <warpcore/counting_hash_table.hpp>
 
+#include <mutex>
+
 using namespace warpcore;
 using wc_capi::as_stream;
 using wc_capi::last_error;
@@ -25,6 +27,9 @@ struct CountingApi {
     virtual Status peek_status(hipStream_t) = 0;
     virtual Status pop_status(hipStream_t) = 0;
     virtual unsigned cg() = 0;
+    // one combining pass in caller-provided buffers (bytes_for(cap) bytes at
+    // base), no allocation and no host round trip
+    virtual void insert_async(const K*, index_t, hipStream_t, index_t, Status*, void* base, index_t cap) = 0;
 };
 
 template<class K, unsigned CG>
@@ -54,6 +59,11 @@ struct CountingImpl final : CountingApi<K> {
     Status peek_status(hipStream_t s) override { return t.peek_status(s); }
     Status pop_status(hipStream_t s) override { return t.pop_status(s); }
     unsigned cg() override { return CG; }
+    void insert_async(const K* k, index_t n, hipStream_t s, index_t pl, Status* st, void* base, index_t cap) override {
+        const auto buf = table_t::CombineBuffers::carve(base, cap);
+        if (st) t.template insert_combined_async<status_handlers::ReturnStatus>(k, n, buf, s, pl, st);
+        else t.insert_combined_async(k, n, buf, s, pl);
+    }
 };
 
 template<class K>
@@ -142,3 +152,61 @@ CountingApi<K>* make(index_t min_cap, K seed, uint64_t max_count, bool no_init, 
 
 WC_COUNTING_API(u32, uint32_t)
 WC_COUNTING_API(u64, uint64_t)
+
+// ---------------------------------------------------------------------------
+// Insert without allocation (no counterpart in the reference): the combining
+// buffers live in `scratch`, a file-scope device array of fixed size that
+// every GPU the library runs on holds until the process ends. Its device
+// address is taken once per device with hipGetSymbolAddress (host code must
+// never pass the array's name to a kernel: on the host it is a stand-in, not
+// a device address). Calls on one GPU share the array, so calls that overlap
+// on one GPU (different streams, or other host threads) race; calls issued
+// one after another on one stream do not. Everything the passes need zeroed
+// (the entry count) is cleared by a fill kernel at the start of each call.
+
+namespace {
+
+constexpr index_t kNoAllocOccurrences = index_t(1) << 20;
+using noalloc_buffers = CountingHashTable<std::uint64_t, std::uint64_t>::CombineBuffers;
+constexpr index_t kNoAllocBytes = noalloc_buffers::bytes_for(kNoAllocOccurrences);
+constexpr int kMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+
+// Device address of `scratch` on the current device (nullptr on failure).
+void* scratch_address() noexcept {
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    if (addr[dev] == nullptr) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return nullptr;
+        addr[dev] = p;
+    }
+    return addr[dev];
+}
+
+}  // namespace
+
+extern "C" int wc_counting_u64_insert_noalloc(void* h, const uint64_t* keys, uint64_t n, void* stream, uint64_t pl,
+                                              uint32_t* status_out) {
+    if (!h) return wc_capi::kBadHandle;
+    if (n > uint64_t(kNoAllocOccurrences)) return wc_capi::kTooLarge;
+    auto* t = static_cast<CountingApi<uint64_t>*>(h);
+    if (n < 4096 || detail::counting_combine() != 1) {   // as wc_counting_u64_insert: no combining buffers
+        t->insert(keys, n, as_stream(stream), pl, reinterpret_cast<Status*>(status_out));
+        return last_error();
+    }
+    void* base = scratch_address();
+    if (base == nullptr) return wc_capi::kNoScratch;
+    t->insert_async(keys, n, as_stream(stream), pl, reinterpret_cast<Status*>(status_out), base, kNoAllocOccurrences);
+    return last_error();
+}
+
+extern "C" uint64_t wc_counting_u64_insert_noalloc_max(void) { return kNoAllocOccurrences; }
