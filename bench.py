@@ -57,15 +57,31 @@ def algorithmic_bytes(op: str, kb: int, vb: int) -> int:
     raise ValueError(op)
 
 
-def read_traffic(kernel_tag: str, log2n: int = 28):
-    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, scaled by a copy kernel of known size), if a summary for this
-    workload (2^28 keys) is committed under profiles/."""
+def read_traffic(family: str, log2n: int = 28):
+    """HBM bytes per launch of kernel `family` (insert_owner, retrieve_owner,
+    ...) on this workload (2^28 keys), measured by rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, each scaled by a copy kernel of known size), if
+    profiles/svht_u64_lf09_traffic.md holds a figure for that kernel; else
+    None (a figure of another kernel is never used)."""
     f = REPO / "profiles" / "svht_u64_lf09_traffic.md"
     if log2n != 28 or not f.exists():
         return None
-    m = re.search(rf"^traffic_bytes_per_launch\.{kernel_tag}\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
+    m = re.search(rf"^traffic_bytes_per_launch\.{family}\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
     return float(m.group(1)) if m else None
+
+
+def kernel_family(lib, op: str) -> str:
+    """The bulk kernel a SingleValueHashTable<u64,u64> (cg 8) call of `op`
+    launches under the current kernel selection (wc_get_group_kernel)."""
+    k = lib.wc_get_group_kernel()
+    if op == "insert":
+        return "insert_owner" if k & 128 else ("insert_refill" if k & 8 else
+                                                ("insert_lockstep" if k & 1 else "insert_batched"))
+    if k & 64:
+        return "retrieve_owner"
+    if k & 4:
+        return "retrieve_refill_pipe" if k & 16 else "retrieve_refill"
+    return "retrieve_lockstep" if k & 2 else "retrieve_batched"
 
 
 def probes_lib():
@@ -528,33 +544,68 @@ def run_mvht(g: Gpu, log2n: int = 26, mean: int = 8, reps: int = 3):
     return res
 
 
-def cpu_baseline(log2n: int = 24, reps: int = 3):
-    """OpenMP table with the same hashing, probing and CAS protocol, on a
-    2^log2n-key sample of the same workload (u64/u64, LF 0.9, 50 % hits)."""
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _omp_pass(oracle, kb: int, n: int, min_cap: int, cg: int, hit_frac: float, seed: int):
+    """One init + insert + retrieve pass of the OpenMP table: n distinct keys
+    (fmix of a counter, sentinels removed), lookups of n keys of which
+    hit_frac are stored. Returns (insert s, retrieve s, capacity)."""
     import numpy as np
-    import oracle
     lib = oracle.lib()
-    n = 1 << log2n
-    idx = np.arange(0x1000, 0x1000 + n + n // 2, dtype=np.uint64)
+    dt = np.uint64 if kb == 64 else np.uint32
+    absent_n = int(n * (1 - hit_frac)) if hit_frac < 1 else n
+    idx = np.arange(0x1000, 0x1000 + n + absent_n + n // 64 + 1024, dtype=np.uint64)
     allk = np.empty_like(idx)
     lib.wco_fmix64_array(oracle._p(idx), oracle._p(allk), len(idx))
-    allk = allk[(allk != 0) & (allk != np.uint64(2**64 - 1))]
-    keys, absent = allk[:n], allk[n:n + n // 2]
-    vals = np.random.default_rng(0xC2).integers(0, 2**63, n, dtype=np.uint64)
-    queries = np.empty(n, np.uint64)
-    queries[0::2], queries[1::2] = keys[: n // 2], absent
-    out = np.zeros(n, np.uint64)
-    t = oracle.OmpTable(math.ceil(n / 0.9), 64, 64, cg=8)
-    best = float("inf")
-    for _ in range(reps):
-        t.init()
-        ti, _ = t.insert(keys, vals, status=False)
-        tr, _ = t.retrieve(queries, out, status=False)
-        best = min(best, ti + tr)
-    return dict(value=round(2 * n / best / 1e9, 4), unit="Gops/s", cores=oracle.omp_threads(), kind="port",
-                sample=f"u64/u64 OpenMP table (same fmix64 double hashing, cg-8 windows, std::atomic CAS), "
-                       f"2^{log2n} unique keys at LF 0.9 then 2^{log2n} lookups at 50% hits; "
-                       f"best of {reps} insert+retrieve passes")
+    if kb == 32:
+        allk = np.unique(allk.astype(np.uint32))   # distinct after the cut to 32 bits
+        np.random.default_rng(seed).shuffle(allk)
+    allk = allk[(allk != 0) & (allk != dt(np.iinfo(dt).max))].astype(dt)
+    keys, absent = allk[:n], allk[n:n + absent_n]
+    vals = np.random.default_rng(seed).integers(0, np.iinfo(dt).max, n, dtype=dt)
+    if hit_frac < 1:
+        hits = keys[: n - absent_n]
+        queries = np.empty(len(hits) + len(absent), dt)
+        queries[0::2], queries[1::2] = hits, absent
+    else:
+        queries = np.concatenate([keys, absent])
+    out = np.zeros(len(queries), dt)
+    t = oracle.OmpTable(min_cap, kb, kb, cg=cg)
+    t.init()
+    ti, _ = t.insert(keys, vals, status=False)
+    tr, _ = t.retrieve(queries, out, status=False)
+    return ti, tr, len(queries), t.capacity()
+
+
+def cpu_baseline():
+    """The OpenMP table (same hashing, probing, CAS protocol and statuses) on
+    the host cores at the configurations' full sizes: SVHT<u64,u64> with 2^28
+    keys at LF 0.9 and 2^28 lookups at 50 % hits (the metric's workload), and
+    SVHT<u32,u32> with 2^20 keys at capacity 2^20/0.8 and lookups of the 2^20
+    keys plus 2^20 absent ones. One pass each (about 10-20 s of CPU work)."""
+    import oracle
+    n2 = 1 << 28
+    ti, tr, q2, cap2 = _omp_pass(oracle, 64, n2, math.ceil(n2 / 0.9), 8, 0.5, 0xC2)
+    n1 = 1 << 20
+    ti1, tr1, q1, cap1 = _omp_pass(oracle, 32, n1, math.ceil(n1 / 0.8), 8, 1.0, 0xC1)
+    threads = oracle.omp_threads()
+    return dict(value=round((n2 + q2) / (ti + tr) / 1e9, 4), unit="Gops/s", cores=threads, kind="port",
+                cpu=_cpu_model(), nproc=os.cpu_count(),
+                insert_gops=round(n2 / ti / 1e9, 4), retrieve_gops=round(q2 / tr / 1e9, 4),
+                c1_u32u32_2p20=dict(gops=round((n1 + q1) / (ti1 + tr1) / 1e9, 4), capacity=cap1,
+                                    insert_gops=round(n1 / ti1 / 1e9, 4), retrieve_gops=round(q1 / tr1 / 1e9, 4)),
+                sample=f"OpenMP u64/u64 table ({threads} threads; fmix64 double hashing over cg-8 line windows, "
+                       f"std::atomic CAS): the full workload, 2^28 unique keys at LF 0.9 ({cap2} slots) then "
+                       f"2^28 lookups at 50% hits, one pass; c1: u32/u32, 2^20 keys, capacity 2^20/0.8, "
+                       f"lookups of 2^20 stored + 2^20 absent keys")
 
 
 def main():
@@ -570,7 +621,6 @@ def main():
     ap.add_argument("--no-structures", action="store_true",
                     help="skip the 2^20-key u32 table, hash set, bloom filter and multi-value table")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-log2-keys", type=int, default=24)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (PMC calibration)")
     args = ap.parse_args()
@@ -635,20 +685,22 @@ def main():
     achieved = per_op[dom] * n / (kt[dom] * 1e-3) / 1e9
     other = "retrieve" if dom == "insert" else "insert"
     achieved_other = per_op[other] * n / (kt[other] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": f"svht_u64u64_{dom}", "achieved": round(achieved, 1),
+    fam = {op: kernel_family(g.lib, op) for op in ("insert", "retrieve")}
+    roofline = {"bound": "hbm", "kernel": f"{fam[dom]} [SVHT u64/u64 cg 8]", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": read_traffic(dom, args.log2_keys),
+                "traffic": read_traffic(fam[dom], args.log2_keys),
+                "traffic_source": "profiles/svht_u64_lf09_traffic.md",
                 "bytes_per_op": per_op[dom], "ops_per_launch": n, "avg_kernel_ms": round(kt[dom], 4),
-                "other_kernel": {"kernel": f"svht_u64u64_{other}", "achieved": round(achieved_other, 1),
+                "other_kernel": {"kernel": f"{fam[other]} [SVHT u64/u64 cg 8]", "achieved": round(achieved_other, 1),
                                  "frac": round(achieved_other / HBM_PEAK_GBS, 4), "bytes_per_op": per_op[other],
                                  "avg_kernel_ms": round(kt[other], 4),
-                                 "traffic": read_traffic(other, args.log2_keys)}}
+                                 "traffic": read_traffic(fam[other], args.log2_keys)}}
     # practical ceiling for random whole lines at the headline table's size
     lr = run_line_roofline(g, r64["table_bytes"], claim=True)
     # inserts against the read-then-claim rate (one window and one CAS per key at best)
     lr["insert_vs_claim"] = round(n / (kt["insert"] * 1e-3) / 1e9 / lr["claim_glines_s"], 4)
     for op in ("retrieve", "insert"):
-        tr = read_traffic(op, args.log2_keys)
+        tr = read_traffic(fam[op], args.log2_keys)
         if tr:   # measured HBM bytes per op in 128-byte line equivalents (reads and write-backs)
             lines_per_op = tr / LINE / n
             lr[f"{op}_lines_per_op"] = round(lines_per_op, 3)
@@ -657,7 +709,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
-        cpu = cpu_baseline(args.cpu_log2_keys)
+        cpu = cpu_baseline()
         log(f"[bench] cpu: {cpu}")
 
     out = {

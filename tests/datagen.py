@@ -81,14 +81,17 @@ class kernel_shape:
     """Context manager selecting the bulk-kernel shape: 'lanes' (one lane per
     key), 'groups' (cooperative groups, streaming kernels), 'lockstep'
     (cooperative groups, lockstep kernels), 'refill' (cooperative groups,
-    refill inserts and lookups) or 'pipe' (the refill kernels pipelined:
-    each slot's result consumed as soon as its own request returns), by
+    refill inserts and lookups), 'pipe' (refill inserts, pipelined refill
+    lookups: each slot's result consumed as soon as its own request
+    returns) or
+    'owner' (lane-owner kernels: every lane owns one key, groups read their
+    owners' windows in turn), by
     moving the table-size limit of
     the lane-per-key path and choosing the group kernels."""
 
     def __init__(self, shape: str):
         self.limit = (1 << 62) if shape == "lanes" else 0
-        self.group_kernel = {"groups": 0, "refill": 12, "pipe": 60}.get(shape, 3)
+        self.group_kernel = {"groups": 0, "refill": 12, "pipe": 28, "owner": 220}.get(shape, 3)
 
     def __enter__(self):
         from warpcore_amd import _capi

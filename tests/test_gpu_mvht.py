@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 # bulk inserts and lookups: one key per group at a time ("groups") or
 # several keys in flight per group ("refill", the default)
-BULK_KERNELS = pytest.mark.parametrize("kernels", ["groups", "refill", "pipe"])
+BULK_KERNELS = pytest.mark.parametrize("kernels", ["groups", "refill", "pipe", "owner"])
 
 S_NONE, S_PLE, S_INVALID_KEY, S_DUP, S_NOT_FOUND = 0, 2, 8, 16, 32
 S_DRY_RUN, S_PHASE, S_MAXV = 1 << 9, 1 << 10, 1 << 11

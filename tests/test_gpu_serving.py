@@ -296,9 +296,8 @@ def test_server_config_shape_matches_launch_per_request(cuda):
 def test_server_insert_erase_cycles_stay_admitted(cuda):
     """Erase leaves tombstones that keep their slots; an insert that no
     longer fits the budget has its table compacted (live pairs re-inserted)
-    and is admitted. Repeated inserts of stored keys claim nothing: their
-    reservation is given back, and one that does not fit the budget is
-    admitted after a lookup shows its keys are stored. Each round: lookups
+    and is admitted. Repeated inserts of stored keys claim nothing: the
+    recount of the table's claimed slots admits them. Each round: lookups
     see exactly the live keys."""
     from warpcore_amd.tables import Server
     srv = Server([1 << 17], 64, 64)

@@ -34,6 +34,13 @@
 namespace warpcore {
 namespace detail {
 
+// Cores of multi-value tables (static constexpr bool multi_value = true)
+// change the insert protocol: every pair takes a new slot, the key's pairs
+// met on the way are counted (multi_value_hash_table.cuh:200-289).
+template<class C, class = void> struct is_multi_value_core : std::false_type {};
+template<class C>
+struct is_multi_value_core<C, std::void_t<decltype(C::multi_value)>> : std::bool_constant<C::multi_value> {};
+
 // Window iterator state of one key: current window, step (double hashing:
 // fixed; quadratic: growing; linear: 1) and windows left.
 template<class ProbingScheme>

@@ -175,12 +175,15 @@ __global__ __launch_bounds__(256) void erase(const typename Core::key_type* __re
 // Number of slots holding a valid key: per-lane counts, one LDS sum per
 // block and one atomicAdd per block (gpu_engine.cuh:378-417).
 template<class Core>
-__global__ __launch_bounds__(256) void size(index_t* __restrict__ num_out, Core core) {
+__global__ __launch_bounds__(256) void size(index_t* __restrict__ num_out, Core core, bool claimed = false) {
     __shared__ index_t partial[256 / 64];
     const index_t stride = index_t(gridDim.x) * blockDim.x;
     index_t count = 0;
-    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < core.capacity(); i += stride)
-        count += core.is_valid_key(core.key_at(i));
+    // valid keys, or (claimed) every slot that is not EMPTY: keys and tombstones
+    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < core.capacity(); i += stride) {
+        const auto k = core.key_at(i);
+        count += claimed ? !Core::is_empty_key(k) : core.is_valid_key(k);
+    }
     count = tile<64>::reduce_add(count);
     if (lane_id() == 0) partial[threadIdx.x / 64] = count;
     __syncthreads();

@@ -538,16 +538,22 @@ WC_H unsigned grid_for_lockstep(index_t items) noexcept {
 // (process-wide setting): bit 0 set = lockstep inserts, bit 1 set =
 // lockstep lookups and erases, bit 2 set = refill lookups (refill_probing.hpp,
 // before bit 1), bit 3 set = refill inserts (before bit 0); a clear bit
-// selects the streaming state machine. Bits 4 / 5 pipeline the refill
-// lookups / inserts (each slot's request consumed as soon as it returns).
-// The default, refill inserts and pipelined refill lookups, is the fastest
-// measured on MI355X (lockstep batches wait for their longest probe
-// sequence, which at load factor 0.9 is many windows; refill slots do not
-// wait; pipelined lookups 16.1 -> 18.8 Gops/s on SVHT<u64,u64> with 2^28
-// keys at LF 0.9, 18.9 -> 23.2 on <u32,u32>; pipelined inserts 11.9 -> 11.3
-// and 11.0 -> 11.7).
+// selects the streaming state machine. Bit 4 pipelines the refill lookups
+// (each slot's request consumed as soon as it returns). Bits 6 / 7 select
+// the lane-owner lookups / inserts for single-value tables with windows of
+// 2..16 slots (owner_probing.hpp), ahead of the refill kernels. The
+// default, lane-owner inserts and lookups, else refill inserts and
+// pipelined refill lookups, is the fastest measured on MI355X (Gops/s,
+// 2^28 keys at LF 0.9, lookups at 50 % hits; insert / lookup):
+//   SVHT<u64,u64> cg 8:  refill 12.0 / 16.1, pipelined lookups 16.6-18.8,
+//                        lane owners 15.9 / 17.5
+//   SVHT<u32,u32> cg 16: refill 11.0 / 18.9, pipelined lookups 20.5-23.2,
+//                        lane owners 18.6 / 23.5
+// (lockstep batches wait for their longest probe sequence, which at load
+// factor 0.9 is many windows; refill slots do not wait; a pipelined refill
+// insert measured 11.3 and 11.7 and was dropped).
 inline std::atomic<int>& group_kernel_setting() noexcept {
-    static std::atomic<int> k{28};
+    static std::atomic<int> k{220};
     return k;
 }
 WC_H int group_kernel() noexcept { return group_kernel_setting().load(std::memory_order_relaxed); }
@@ -556,9 +562,12 @@ WC_H bool lockstep_inserts() noexcept { return (group_kernel() & 1) != 0; }
 WC_H bool lockstep_lookups() noexcept { return (group_kernel() & 2) != 0; }
 WC_H bool refill_lookups() noexcept { return (group_kernel() & 4) != 0; }
 WC_H bool refill_inserts() noexcept { return (group_kernel() & 8) != 0; }
-// bit 4: pipelined refill lookups, bit 5: pipelined refill inserts
+// bit 4: pipelined refill lookups
 WC_H bool pipelined_lookups() noexcept { return (group_kernel() & 16) != 0; }
-WC_H bool pipelined_inserts() noexcept { return (group_kernel() & 32) != 0; }
+// bit 6: lane-owner lookups, bit 7: lane-owner inserts (owner_probing.hpp;
+// single-value tables; before the refill bits)
+WC_H bool owner_lookups() noexcept { return (group_kernel() & 64) != 0; }
+WC_H bool owner_inserts() noexcept { return (group_kernel() & 128) != 0; }
 
 template<class Core, class StatusHandler, int U>
 WC_H void launch_erase_lockstep(const typename Core::key_type* keys_in, index_t num_in, const Core& core,

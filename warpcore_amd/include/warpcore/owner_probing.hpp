@@ -1,0 +1,401 @@
+// Lane-owner probing: in a group of CG lanes (CG = slots per probe window,
+// one 128-byte line for 8- and 16-byte slots) every lane owns one key. Each
+// round the group reads the windows of its CG keys one after another, all
+// lanes of the group reading one window per load instruction (one address
+// translation per line), so a wavefront has 64 keys in flight. A key's
+// state (key, value, window, step, phase) lives in its owner lane only:
+// per round and window the wavefront spends one LDS broadcast read (the
+// owner's key and window, published in LDS), one load, two compares with
+// 64-lane ballots and a bit-field extract; the owner then decides alone,
+// and claims a slot with its own CAS. Free lanes take the wave's next key
+// (prefetched one key ahead).
+//
+// The windows of a key are read in the order of the one-key protocols
+// (single_value_hash_table.cuh:742-823 for insert, :267-307 for retrieve):
+// a hit in a window is a duplicate (insert: the value is overwritten) or
+// the key's value (retrieve); an insert claims the lowest EMPTY lane of the
+// first window that has one and tries that window's other EMPTY lanes in
+// order when a CAS loses; a lookup ends as missing at a window with an
+// EMPTY lane; the window budget ends a key with probing_length_exceeded.
+// Results and statuses are those of the other kernel families; keys of one
+// batch interleave as keys of different groups do in the reference.
+#pragma once
+
+#include "batched_probing.hpp"
+#include "engine.hpp"
+#include "tile.hpp"
+
+namespace warpcore {
+namespace detail {
+
+// What an owner publishes for its group each round: its key, its current
+// window (kNoWindow: not reading a window this round) and (lookups) its
+// position in the range.
+template<class K, class W>
+struct alignas(16) OwnerEntry {
+    K key;
+    W w;
+    std::uint32_t pos;
+};
+
+template<class W> constexpr W kNoWindow = W(~W(0));
+
+WC_D unsigned lanes_below(std::uint64_t m) noexcept {
+    return __builtin_amdgcn_mbcnt_hi(unsigned(m >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(m), 0u));
+}
+
+// Bits of the caller's group in the 64-lane mask m.
+template<unsigned CG>
+WC_D std::uint64_t group_field(std::uint64_t m, unsigned shift) noexcept {
+    if constexpr (CG == 64) return m;
+    else return (m >> shift) & ((std::uint64_t(1) << CG) - 1);
+}
+
+// Inserts of keys_in[first .. end) by the calling wavefront (all 64 lanes
+// call this with the same range). `lds` holds 64 entries of this wave.
+// Single-value cores only (insert_refill_range covers multi-value cores).
+template<class Core, class StatusHandler, class W, bool HasValues>
+WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in,
+                             const typename Core::value_type* __restrict__ values_in, index_t first, index_t end,
+                             const Core& core, index_t budget,
+                             typename StatusHandler::base_type* __restrict__ status_out,
+                             OwnerEntry<typename Core::key_type, W>* lds, DeferredJoin& join) {
+    using K = typename Core::key_type;
+    using V = typename Core::value_type;
+    using Probe = typename Core::insert_probe_type;
+    using Cursor = WindowCursor<typename Core::probing_scheme_type>;
+    using Entry = OwnerEntry<K, W>;
+    using Bits = typename std::conditional<(Core::cg_size() <= 32), std::uint32_t, std::uint64_t>::type;
+    constexpr unsigned CG = unsigned(Core::cg_size());
+    static_assert(CG >= 2 && CG <= 64 && (CG & (CG - 1)) == 0, "groups of 2..64 lanes");
+    static_assert(!is_multi_value_core<Core>::value, "single-value cores only");
+    constexpr bool keep_seen = !std::is_same<Probe, K>::value;
+    constexpr unsigned kFree = 0, kLoad = 1, kClaim = 2;
+    const unsigned lane = lane_id();
+    const unsigned rank = lane & (CG - 1);
+    const unsigned shift = lane & ~(CG - 1);
+    auto ballot = [](bool p) -> std::uint64_t { return __builtin_amdgcn_ballot_w64(p); };
+
+    const probing_schemes::Geometry& geo = core.geometry();
+    const W p = W(geo.windows);
+    auto* __restrict__ slots = core.slot_ptr();
+    if (end <= first) return;
+    if (!core.ready()) {
+        for (index_t i = first + lane; i < end; i += 64)
+            StatusHandler::handle(Status::not_initialized(), status_out, i);
+        return;
+    }
+
+    K key = K{};
+    V val = V{};
+    W w = 0, step = 1;
+    std::uint32_t rnd = 0;
+    index_t pos = 0;
+    unsigned phase = kFree;
+    Bits emp = 0;
+    Probe seen = Core::empty_probe();
+    // the lane's next key, loaded one key ahead
+    index_t cursor = first;
+    K nk = K{};
+    V nv = V{};
+    index_t npos = cursor + lane;
+    bool nhas = npos < end;
+    if (nhas) {
+        nk = keys_in[npos];
+        if constexpr (HasValues) nv = values_in[npos];
+    }
+    cursor += 64;
+
+    auto advance = [&]() {   // to the key's next window, or out of windows
+        if (rnd + 1 >= budget) {
+            join(Status::probing_length_exceeded());
+            StatusHandler::handle(Status::probing_length_exceeded(), status_out, pos);
+            phase = kFree;
+        } else {
+            const W nw = w + step;
+            w = nw >= p ? W(nw - p) : nw;
+            rnd += 1;
+            phase = kLoad;
+        }
+    };
+
+    while (true) {
+        // a free lane takes its prefetched key and prefetches the wave's next
+        const bool take = phase == kFree && nhas;
+        if (take) {
+            key = nk;
+            val = nv;
+            pos = npos;
+            if (core.is_valid_key(key)) {
+                Cursor c;
+                c.begin(key, core.hash_seed(), geo);
+                w = W(c.w);
+                step = W(c.step);
+                rnd = 0;
+                phase = kLoad;
+            } else {
+                join(Status::invalid_key());
+                StatusHandler::handle(Status::invalid_key(), status_out, pos);
+            }
+        }
+        const std::uint64_t taken = ballot(take);
+        if (taken != 0) {
+            if (take) {
+                npos = cursor + lanes_below(taken);
+                nhas = npos < end;
+                if (nhas) {
+                    nk = keys_in[npos];
+                    if constexpr (HasValues) nv = values_in[npos];
+                }
+            }
+            cursor += index_t(__builtin_popcountll(taken));
+        }
+        if (ballot(phase != kFree || nhas) == 0) break;
+
+        // publish key and window; every group reads its owners' windows
+        Entry mine;
+        mine.key = key;
+        mine.w = phase == kLoad ? w : kNoWindow<W>;
+        mine.pos = 0;
+        lds[lane] = mine;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        K kj[CG];
+        Probe probe[CG];
+        bool on[CG];
+#pragma unroll
+        for (unsigned j = 0; j < CG; ++j) {
+            const Entry e = lds[shift + j];
+            kj[j] = e.key;
+            on[j] = e.w != kNoWindow<W>;
+            const W at = on[j] ? e.w : W(0);
+            probe[j] = Core::load_insert_probe(slots + index_t(at) * CG + rank);
+        }
+        // claiming owners: CAS on the lowest untried EMPTY lane
+        const unsigned lead = unsigned(ffs64(std::uint64_t(emp)) - 1);
+        typename Core::cas_word old{};
+        if (phase == kClaim) old = core.cas_issue(slots + index_t(w) * CG + lead, seen, key, val);
+        __builtin_amdgcn_wave_barrier();
+
+        // each window's hits and EMPTY lanes, extracted by its owner
+        Bits hit = 0, empty = 0;
+#pragma unroll
+        for (unsigned j = 0; j < CG; ++j) {
+            const K tk = Core::probe_key(probe[j]);
+            const std::uint64_t hm = ballot(on[j] && tk == kj[j]);
+            const std::uint64_t em = ballot(on[j] && Core::is_empty_key(tk));
+            if (rank == j) {
+                hit = Bits(group_field<CG>(hm, shift));
+                empty = Bits(group_field<CG>(em, shift));
+            }
+        }
+
+        if (phase == kLoad) {
+            if (hit != 0) {   // duplicate: the value is overwritten in place
+                Status st = Status::duplicate_key();
+                core.finish_insert(index_t(w) * CG + unsigned(ffs64(std::uint64_t(hit)) - 1), val, false, st, join);
+                join(Status::duplicate_key());
+                StatusHandler::handle(st, status_out, pos);
+                phase = kFree;
+            } else if (empty != 0) {
+                emp = empty;
+                if constexpr (keep_seen) seen = Core::empty_probe();
+                phase = kClaim;
+            } else {
+                advance();
+            }
+        } else if (phase == kClaim) {
+            const int code = core.cas_outcome(old, seen, key);
+            if (code == 1 || code == 2) {   // claimed, or the key was already there
+                Status st = code == 1 ? Status::none() : Status::duplicate_key();
+                core.finish_insert(index_t(w) * CG + lead, val, code == 1 && Core::publishes_value, st, join);
+                if (code == 2) join(Status::duplicate_key());
+                StatusHandler::handle(st, status_out, pos);
+                phase = kFree;
+            } else if (code == 0) {   // another key took the lane: the window's next EMPTY lane
+                emp &= Bits(emp - 1);
+                if (emp == 0) advance();
+            }                         // code 3: the same lane again, with its refreshed word
+        }
+    }
+}
+
+// Lookups of keys_in[first .. end) by the calling wavefront. A hit is
+// written by the lane holding the slot; the owner writes the status.
+template<class Core, class StatusHandler, class W>
+WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_in, index_t first, index_t end,
+                               typename Core::out_type* __restrict__ values_out, const Core& core, index_t budget,
+                               typename StatusHandler::base_type* __restrict__ status_out,
+                               OwnerEntry<typename Core::key_type, W>* lds, DeferredJoin& join) {
+    using K = typename Core::key_type;
+    using O = typename Core::out_type;
+    using Slot = typename Core::probe_slot_type;
+    using Cursor = WindowCursor<typename Core::probing_scheme_type>;
+    using Entry = OwnerEntry<K, W>;
+    constexpr unsigned CG = unsigned(Core::cg_size());
+    static_assert(CG >= 2 && CG <= 64 && (CG & (CG - 1)) == 0, "groups of 2..64 lanes");
+    static_assert(!is_multi_value_core<Core>::value, "single-value cores only");
+    constexpr unsigned kFree = 0, kLoad = 1;
+    const unsigned lane = lane_id();
+    const unsigned rank = lane & (CG - 1);
+    const unsigned shift = lane & ~(CG - 1);
+    auto ballot = [](bool p) -> std::uint64_t { return __builtin_amdgcn_ballot_w64(p); };
+
+    const probing_schemes::Geometry& geo = core.geometry();
+    const W p = W(geo.windows);
+    const Slot* __restrict__ lane_slots = core.slot_ptr() + rank;
+    if (end <= first) return;
+    if (!core.ready()) {
+        for (index_t i = first + lane; i < end; i += 64)
+            StatusHandler::handle(Status::not_initialized(), status_out, i);
+        return;
+    }
+    O miss_value{};
+    const Status miss_status = core.missing(miss_value);
+
+    K key = K{};
+    W w = 0, step = 1;
+    std::uint32_t rnd = 0;
+    index_t pos = 0;
+    unsigned phase = kFree;
+    index_t cursor = first;
+    K nk = K{};
+    index_t npos = cursor + lane;
+    bool nhas = npos < end;
+    if (nhas) nk = keys_in[npos];
+    cursor += 64;
+
+    while (true) {
+        const bool take = phase == kFree && nhas;
+        if (take) {
+            key = nk;
+            pos = npos;
+            if (core.is_valid_key(key)) {
+                Cursor c;
+                c.begin(key, core.hash_seed(), geo);
+                w = W(c.w);
+                step = W(c.step);
+                rnd = 0;
+                phase = kLoad;
+            } else {
+                join(Status::invalid_key());
+                StatusHandler::handle(Status::invalid_key(), status_out, pos);
+            }
+        }
+        const std::uint64_t taken = ballot(take);
+        if (taken != 0) {
+            if (take) {
+                npos = cursor + lanes_below(taken);
+                nhas = npos < end;
+                if (nhas) nk = keys_in[npos];
+            }
+            cursor += index_t(__builtin_popcountll(taken));
+        }
+        if (ballot(phase != kFree || nhas) == 0) break;
+
+        Entry mine;
+        mine.key = key;
+        mine.w = phase == kLoad ? w : kNoWindow<W>;
+        mine.pos = std::uint32_t(pos - first);
+        lds[lane] = mine;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        K kj[CG];
+        std::uint32_t pj[CG];
+        Slot s[CG];
+        bool on[CG];
+#pragma unroll
+        for (unsigned j = 0; j < CG; ++j) {
+            const Entry e = lds[shift + j];
+            kj[j] = e.key;
+            pj[j] = e.pos;
+            on[j] = e.w != kNoWindow<W>;
+            const W at = on[j] ? e.w : W(0);
+            s[j] = load_slot(lane_slots + index_t(at) * CG);
+        }
+        __builtin_amdgcn_wave_barrier();
+        bool hit = false, empty = false;
+#pragma unroll
+        for (unsigned j = 0; j < CG; ++j) {
+            const K tk = Core::key_of(s[j]);
+            const bool h = on[j] && tk == kj[j];
+            if (h) values_out[first + pj[j]] = core.found(Core::value_of(s[j]));   // the slot's lane writes the hit
+            const std::uint64_t hm = ballot(h);
+            const std::uint64_t em = ballot(on[j] && Core::is_empty_key(tk));
+            if (rank == j) {
+                hit = group_field<CG>(hm, shift) != 0;
+                empty = group_field<CG>(em, shift) != 0;
+            }
+        }
+        if (phase == kLoad) {
+            if (hit) {
+                StatusHandler::handle(Status::none(), status_out, pos);
+                phase = kFree;
+            } else if (empty) {
+                join(miss_status);
+                StatusHandler::handle(miss_status, status_out, pos);
+                if (!miss_status.has_any()) values_out[pos] = miss_value;
+                phase = kFree;
+            } else if (rnd + 1 >= budget) {
+                join(Status::probing_length_exceeded());
+                StatusHandler::handle(Status::probing_length_exceeded(), status_out, pos);
+                phase = kFree;
+            } else {
+                const W nw = w + step;
+                w = nw >= p ? W(nw - p) : nw;
+                rnd += 1;
+            }
+        }
+    }
+}
+
+}  // namespace detail
+
+namespace kernels {
+
+// Bulk inserts / lookups by lane owners: wavefront v of the grid takes keys
+// [v * per, (v + 1) * per).
+template<class Core, class StatusHandler, class W, bool HasValues>
+__global__ __launch_bounds__(256) void insert_owner(const typename Core::key_type* __restrict__ keys_in,
+                                                    const typename Core::value_type* __restrict__ values_in,
+                                                    index_t num_in, Core core, index_t probing_length,
+                                                    typename StatusHandler::base_type* __restrict__ status_out,
+                                                    const index_t* __restrict__ num_dev) {
+    __shared__ detail::OwnerEntry<typename Core::key_type, W> lds[256];
+    detail::DeferredJoin join;
+    if (num_dev != nullptr && *num_dev < num_in) num_in = *num_dev;
+    const index_t wave = (index_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+    const index_t waves = (index_t(gridDim.x) * blockDim.x) / 64;
+    const index_t per = SDIV(num_in, waves);
+    const index_t first = wave * per < num_in ? wave * per : num_in;
+    const index_t end = first + per < num_in ? first + per : num_in;
+    const index_t budget = probing_schemes::detail::window_budget(probing_length, Core::cg_size(),
+                                                                  core.geometry().windows);
+    detail::insert_owner_range<Core, StatusHandler, W, HasValues>(keys_in, values_in, first, end, core, budget,
+                                                                  status_out, lds + (threadIdx.x & ~63u), join);
+    join.flush(core.status_word());
+}
+
+template<class Core, class StatusHandler, class W>
+__global__ __launch_bounds__(256) void retrieve_owner(const typename Core::key_type* __restrict__ keys_in,
+                                                      index_t num_in, typename Core::out_type* __restrict__ values_out,
+                                                      Core core, index_t probing_length,
+                                                      typename StatusHandler::base_type* __restrict__ status_out) {
+    __shared__ detail::OwnerEntry<typename Core::key_type, W> lds[256];
+    detail::DeferredJoin join;
+    const index_t wave = (index_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+    const index_t waves = (index_t(gridDim.x) * blockDim.x) / 64;
+    const index_t per = SDIV(num_in, waves);
+    const index_t first = wave * per < num_in ? wave * per : num_in;
+    const index_t end = first + per < num_in ? first + per : num_in;
+    const index_t budget = probing_schemes::detail::window_budget(probing_length, Core::cg_size(),
+                                                                  core.geometry().windows);
+    detail::retrieve_owner_range<Core, StatusHandler, W>(keys_in, first, end, values_out, core, budget, status_out,
+                                                         lds + (threadIdx.x & ~63u), join);
+    join.flush(core.status_word());
+}
+
+}  // namespace kernels
+}  // namespace warpcore

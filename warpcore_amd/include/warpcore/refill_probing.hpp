@@ -27,6 +27,7 @@
 
 #include "batched_probing.hpp"
 #include "engine.hpp"
+#include "owner_probing.hpp"
 #include "tile.hpp"
 
 namespace warpcore {
@@ -50,12 +51,6 @@ template<unsigned CG> constexpr int refill_insert_keys() noexcept {
     return CG < WARPCORE_REFILL_INSERT_KEYS ? int(CG) : WARPCORE_REFILL_INSERT_KEYS;
 }
 
-// Cores of multi-value tables (static constexpr bool multi_value = true)
-// change the insert protocol: every pair takes a new slot, the key's pairs
-// met on the way are counted (multi_value_hash_table.cuh:200-289).
-template<class C, class = void> struct is_multi_value_core : std::false_type {};
-template<class C>
-struct is_multi_value_core<C, std::void_t<decltype(C::multi_value)>> : std::bool_constant<C::multi_value> {};
 
 // One staged key: the key, its first window and its window step.
 template<class K, class W>
@@ -324,10 +319,8 @@ struct alignas(16) RefillInsertEntry {
 // when a CAS claims a slot (none if no value of the key was met, else
 // duplicate_key; not joined) or when the count reaches core.max_values
 // (duplicate_key + max_values_for_key_reached, joined). A claim with no
-// earlier value adds one to *new_keys. Single-value cores add one to
-// *new_keys per claimed slot (the keys reported `none`), in the group's
-// rank-0 lane.
-template<class Core, class StatusHandler, int U, class W, bool HasValues, bool Pipe = false>
+// earlier value adds one to *new_keys.
+template<class Core, class StatusHandler, int U, class W, bool HasValues>
 WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_in,
                               const typename Core::value_type* __restrict__ values_in, index_t first, index_t end,
                               const Core& core, index_t budget,
@@ -390,7 +383,7 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
         live[u] = 0;
         claim[u] = 0;
     }
-    // requests of the slots (pipelined: in flight from the previous pass)
+    // requests of the slots in the current round
     Probe fresh[U];
     typename Core::cas_word old[U];
     bool lead[U];
@@ -413,11 +406,6 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
         const index_t i0 = first + (rank < len ? rank : 0);
         ahead = keys_in[i0];
         if constexpr (HasValues) ahead_value = values_in[i0];
-    }
-    if constexpr (Pipe) {   // every slot has a load in flight (window 0) from the start, issued
-                            // after `ahead`, so the first staging need not wait for the slots
-#pragma unroll
-        for (int u = 0; u < U; ++u) fresh[u] = Core::load_insert_probe(slots + rank);
     }
 
     while (true) {
@@ -549,7 +537,6 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
                     core.finish_insert(index_t(w[u]) * CG + at_lane, val[u], mine(won) && Core::publishes_value, st,
                                        join);
                     StatusHandler::handle(st, status_out, first + pos[u]);
-                    if (mine(won) && new_keys != nullptr) *new_keys += 1;
                 }
             }
             // slots moving to the next window, or out of windows
@@ -572,34 +559,19 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
             claim[u] = (claim[u] | gemp) & ~(done | adv);
             live[u] &= ~(done | last);
         };
+        // one round: refill, every slot's load or CAS, then every result
         std::uint64_t any_live = 0;
-        if constexpr (!Pipe) {
-            // one round: refill, every slot's load or CAS, then every result
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                take_slot(u);
-                any_live |= live[u];
-            }
-            if (any_live == 0 && ballot(avail != 0 || next < len) == 0) break;
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int u = 0; u < U; ++u) issue_slot(u);
-#pragma unroll
-            for (int u = 0; u < U; ++u) process_slot(u);
-        } else {
-            // pipelined: slot u's load or CAS (issued one pass earlier) is
-            // consumed and its next one issued before slot u + 1 is looked
-            // at, so a pass waits only for the oldest request in flight
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (live[u] != 0) process_slot(u);
-                take_slot(u);
-                any_live |= live[u];
-                issue_slot(u);
-            }
-            __builtin_amdgcn_wave_barrier();
-            if (any_live == 0 && ballot(avail != 0 || next < len) == 0) break;
+        for (int u = 0; u < U; ++u) {
+            take_slot(u);
+            any_live |= live[u];
         }
+        if (any_live == 0 && ballot(avail != 0 || next < len) == 0) break;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int u = 0; u < U; ++u) issue_slot(u);
+#pragma unroll
+        for (int u = 0; u < U; ++u) process_slot(u);
     }
     if constexpr (multi) {
         if (mine(dups)) join(Status::duplicate_key() + Status::max_values_for_key_reached());
@@ -692,15 +664,12 @@ __global__ __launch_bounds__(256) WC_REFILL_PIPE_OCCUPANCY void retrieve_refill_
 // Bulk inserts. num_dev (nullable): the key count is min(num_in, *num_dev),
 // read on the device, so a caller whose count is produced by an earlier
 // kernel on the stream needs no host round trip (num_in bounds it).
-// claims_out (nullable, single-value cores): the number of slots the call
-// claimed is added to *claims_out (one atomic per wavefront).
-template<class Core, class StatusHandler, int U, class W, bool HasValues, bool Pipe>
+template<class Core, class StatusHandler, int U, class W, bool HasValues>
 __global__ __launch_bounds__(256) WC_REFILL_INSERT_OCCUPANCY void insert_refill(const typename Core::key_type* __restrict__ keys_in,
                                                      const typename Core::value_type* __restrict__ values_in,
                                                      index_t num_in, Core core, index_t probing_length,
                                                      typename StatusHandler::base_type* __restrict__ status_out,
-                                                     const index_t* __restrict__ num_dev,
-                                                     index_t* __restrict__ claims_out) {
+                                                     const index_t* __restrict__ num_dev) {
     constexpr unsigned CG = unsigned(Core::cg_size());
     __shared__ detail::RefillInsertEntry<typename Core::key_type, W, typename Core::value_type> stage[256];
     detail::DeferredJoin join;
@@ -712,15 +681,10 @@ __global__ __launch_bounds__(256) WC_REFILL_INSERT_OCCUPANCY void insert_refill(
     const index_t end = first + per < num_in ? first + per : num_in;
     const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, core.geometry().windows);
     index_t new_keys = 0;
-    detail::insert_refill_range<Core, StatusHandler, U, W, HasValues, Pipe>(
+    detail::insert_refill_range<Core, StatusHandler, U, W, HasValues>(
         keys_in, values_in, first, end, core, budget, status_out, stage + (threadIdx.x & ~(CG - 1)), join, &new_keys);
     join.flush(core.status_word());
-    if constexpr (detail::is_multi_value_core<Core>::value) {
-        core.add_keys(new_keys);
-    } else if (claims_out != nullptr) {
-        const index_t all = tile<64>::reduce_add(new_keys);
-        if (lane_id() == 0 && all != 0) detail::atomic_add(claims_out, all);
-    }
+    if constexpr (detail::is_multi_value_core<Core>::value) core.add_keys(new_keys);
 }
 
 }  // namespace kernels
@@ -743,35 +707,73 @@ WC_H unsigned refill_grid() noexcept {
     return grid;
 }
 
-template<class Core, class StatusHandler, int U, bool HasValues, bool Pipe>
+template<class Core, class StatusHandler, int U, bool HasValues>
 WC_H void launch_insert_refill_as(const typename Core::key_type* keys_in, const typename Core::value_type* values_in,
                                   index_t num_in, const Core& core, index_t probing_length,
                                   typename StatusHandler::base_type* status_out, hipStream_t stream,
-                                  const index_t* num_dev, index_t* claims_out) noexcept {
+                                  const index_t* num_dev) noexcept {
     if (core.geometry().windows < (index_t(1) << 31)) {
-        constexpr auto k = kernels::insert_refill<Core, StatusHandler, U, std::uint32_t, HasValues, Pipe>;
+        constexpr auto k = kernels::insert_refill<Core, StatusHandler, U, std::uint32_t, HasValues>;
         k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out,
-                                                         num_dev, claims_out);
+                                                         num_dev);
     } else {
-        constexpr auto k = kernels::insert_refill<Core, StatusHandler, U, index_t, HasValues, Pipe>;
+        constexpr auto k = kernels::insert_refill<Core, StatusHandler, U, index_t, HasValues>;
         k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out,
-                                                         num_dev, claims_out);
+                                                         num_dev);
     }
+}
+
+template<class Core, class StatusHandler, bool HasValues>
+WC_H void launch_insert_owner(const typename Core::key_type* keys_in, const typename Core::value_type* values_in,
+                              index_t num_in, const Core& core, index_t probing_length,
+                              typename StatusHandler::base_type* status_out, hipStream_t stream,
+                              const index_t* num_dev) noexcept {
+    if (core.geometry().windows < (index_t(1) << 31)) {
+        constexpr auto k = kernels::insert_owner<Core, StatusHandler, std::uint32_t, HasValues>;
+        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out,
+                                                         num_dev);
+    } else {
+        constexpr auto k = kernels::insert_owner<Core, StatusHandler, index_t, HasValues>;
+        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out,
+                                                         num_dev);
+    }
+}
+
+template<class Core, class StatusHandler>
+WC_H void launch_retrieve_owner(const typename Core::key_type* keys_in, index_t num_in,
+                                typename Core::out_type* values_out, const Core& core, index_t probing_length,
+                                typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
+    if (core.geometry().windows < (index_t(1) << 31)) {
+        constexpr auto k = kernels::retrieve_owner<Core, StatusHandler, std::uint32_t>;
+        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
+    } else {
+        constexpr auto k = kernels::retrieve_owner<Core, StatusHandler, index_t>;
+        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
+    }
+}
+
+// Lane-owner kernels apply to single-value cores whose windows are 2..16
+// slots (a 128-byte line of 8- or 16-byte slots at most: a group reads one
+// window per owner, so wider windows would hold 32 or 64 windows per lane).
+template<class Core>
+constexpr bool owner_capable() noexcept {
+    return !is_multi_value_core<Core>::value && Core::cg_size() >= 2 && Core::cg_size() <= 16;
 }
 
 template<class Core, class StatusHandler, int U, bool HasValues>
 WC_H void launch_insert_refill(const typename Core::key_type* keys_in, const typename Core::value_type* values_in,
                                index_t num_in, const Core& core, index_t probing_length,
                                typename StatusHandler::base_type* status_out, hipStream_t stream,
-                               const index_t* num_dev = nullptr, index_t* claims_out = nullptr) noexcept {
-    if (pipelined_inserts())
-        launch_insert_refill_as<Core, StatusHandler, U, HasValues, true>(keys_in, values_in, num_in, core,
-                                                                         probing_length, status_out, stream, num_dev,
-                                                                         claims_out);
-    else
-        launch_insert_refill_as<Core, StatusHandler, U, HasValues, false>(keys_in, values_in, num_in, core,
-                                                                          probing_length, status_out, stream, num_dev,
-                                                                          claims_out);
+                               const index_t* num_dev = nullptr) noexcept {
+    if constexpr (owner_capable<Core>()) {
+        if (owner_inserts()) {
+            launch_insert_owner<Core, StatusHandler, HasValues>(keys_in, values_in, num_in, core, probing_length,
+                                                                status_out, stream, num_dev);
+            return;
+        }
+    }
+    launch_insert_refill_as<Core, StatusHandler, U, HasValues>(keys_in, values_in, num_in, core, probing_length,
+                                                               status_out, stream, num_dev);
 }
 
 template<class Core, class StatusHandler, int U, bool Pipe>
@@ -793,6 +795,13 @@ template<class Core, class StatusHandler, int U>
 WC_H void launch_retrieve_refill(const typename Core::key_type* keys_in, index_t num_in,
                                  typename Core::out_type* values_out, const Core& core, index_t probing_length,
                                  typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
+    if constexpr (owner_capable<Core>()) {
+        if (owner_lookups()) {
+            launch_retrieve_owner<Core, StatusHandler>(keys_in, num_in, values_out, core, probing_length, status_out,
+                                                       stream);
+            return;
+        }
+    }
     constexpr int UP = U == WARPCORE_REFILL_KEYS ? WARPCORE_REFILL_PIPE_KEYS : U;
     if (pipelined_lookups())
         launch_retrieve_refill_as<Core, StatusHandler, UP, true>(keys_in, num_in, values_out, core, probing_length,

@@ -29,19 +29,19 @@
 // max_load * capacity, counted in claimed slots (stored keys and the
 // tombstones erase leaves, which inserts never reuse). An insert of n keys
 // reserves n slots of its table when it is admitted (a key may already be
-// stored, so this is an upper bound); the slots it claims (counted by the
-// insert kernels) turn the reservation into claimed slots, and what is left
-// of it is given back when its last step has run. An insert that does not
-// fit waits until the steps in flight have run; if it still does not fit,
-// the table is compacted when it holds tombstones (its live pairs are read
-// out, the table re-initialised and the pairs inserted again, which drops
-// the tombstones; slot placement changes, results do not: with the default
-// probing length they depend only on the stored pairs while the table has
-// an empty slot, which the budget keeps); if it still does not fit, its keys
-// are looked up, and the ones not stored (an upper bound of the slots it can
-// claim, exact up to repeats inside the request) are its reservation
-// instead of n; an insert that does not fit even then is refused (status
-// out_of_memory for every key).
+// stored, so this is an upper bound). An insert that does not fit waits
+// until the steps in flight have run; then the table's reservation is
+// recounted as its claimed slots (one scan of the table: what the earlier
+// inserts really claimed); if it still does not fit, the table is compacted
+// when it holds tombstones (its live pairs are read out, the table
+// re-initialised and the pairs inserted again, which drops the tombstones;
+// slot placement changes, results do not: with the default probing length
+// they depend only on the stored pairs while the table has an empty slot,
+// which the budget keeps); if it still does not fit, its keys are looked
+// up, and the ones not stored (an upper bound of the slots it can claim,
+// exact up to repeats inside the request) are its reservation instead of n;
+// an insert that does not fit even then is refused (status out_of_memory
+// for every key).
 //
 // Failures: a step whose launch or completion event reports an error fails
 // every request it holds (wait() returns an error code; outputs and statuses
@@ -97,15 +97,13 @@ struct RequestDesc {
 namespace kernels {
 
 // Step preparation (one workgroup): copies the step's live descriptors from
-// the host-pinned staging buffer the scheduler wrote, clears their claim
-// counters, counts each request's work items and writes their exclusive
-// scan (first item of each request).
+// the host-pinned staging buffer the scheduler wrote, counts each request's
+// work items and writes their exclusive scan (first item of each request).
 template<class Desc>
 __global__ __launch_bounds__(256) void serve_prepare(const Desc* __restrict__ staged,
                                                      const std::uint32_t* __restrict__ staged_num, index_t max_reqs,
                                                      index_t item_keys, Desc* __restrict__ reqs,
-                                                     index_t* __restrict__ first, index_t* __restrict__ state,
-                                                     index_t* __restrict__ claims) {
+                                                     index_t* __restrict__ first, index_t* __restrict__ state) {
     const index_t num_raw = reinterpret_cast<const volatile std::uint32_t*>(staged_num)[0];
     const index_t num = num_raw < max_reqs ? num_raw : max_reqs;
     const index_t step_item_keys = reinterpret_cast<const volatile std::uint32_t*>(staged_num)[1];
@@ -117,7 +115,6 @@ __global__ __launch_bounds__(256) void serve_prepare(const Desc* __restrict__ st
         if (r < num) {
             const Desc d = staged[r];
             reqs[r] = d;
-            claims[r] = 0;
             k = SDIV(d.n, item_keys);
         }
         index_t total;
@@ -146,15 +143,13 @@ __global__ __launch_bounds__(256) void serve_prepare(const Desc* __restrict__ st
 // (items hold item_keys keys, so a static assignment balances and needs no
 // shared counter). An item's request is found by binary search in an LDS
 // copy of the requests' first items. Lookups and inserts use the refill
-// kernels' range code, erases the lockstep one. claims[r]: slots the
-// inserts of request r claimed.
+// kernels' range code, erases the lockstep one.
 template<class View, int UR, int UI, int UE>
 __global__ __launch_bounds__(256) WC_SERVE_OCCUPANCY void serve_step(const serving::RequestDesc<typename View::key_type,
                                                                              typename View::value_type>* __restrict__ reqs,
                                                   const index_t* __restrict__ first_item,
                                                   index_t* __restrict__ state, index_t max_reqs, index_t item_keys,
-                                                  const View* __restrict__ views, index_t probing_length,
-                                                  index_t* __restrict__ claims) {
+                                                  const View* __restrict__ views, index_t probing_length) {
     using K = typename View::key_type;
     using W = std::uint32_t;
     constexpr unsigned CG = unsigned(View::cg_size());
@@ -190,12 +185,10 @@ __global__ __launch_bounds__(256) WC_SERVE_OCCUPANCY void serve_step(const servi
             const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, view.geo.windows);
             detail::DeferredJoin join;
             auto* st = reinterpret_cast<Status*>(d.status_out);
-            if (d.op == std::uint32_t(serving::Op::insert)) {
-                index_t claimed = 0;
+            if (d.op == std::uint32_t(serving::Op::insert))
                 detail::insert_refill_range<View, H, UI, W, true>(d.keys, d.values, begin, end, view, budget, st,
-                                                                  my_refill_insert, join, &claimed);
-                if (rank == 0 && claimed != 0) detail::atomic_add(claims + r, claimed);
-            } else if (d.op == std::uint32_t(serving::Op::retrieve))
+                                                                  my_refill_insert, join);
+            else if (d.op == std::uint32_t(serving::Op::retrieve))
                 detail::retrieve_refill_range<View, H, UR, W>(d.keys, begin, end, d.values_out, view, budget, st,
                                                               my_refill, join);
             else
@@ -261,7 +254,6 @@ public:
         bool failed = false;
         bool probed = false;    // its keys were looked up for admission
         index_t need = 0;       // probed: keys of it not stored
-        index_t held = 0;       // reservation not yet claimed
     };
 
     Server(const index_t* table_capacities, index_t num_tables, Config cfg = Config{}) : cfg_(cfg) {
@@ -278,16 +270,15 @@ public:
             }
             budget_.push_back(index_t(double(tables_.back()->capacity()) * cfg_.max_load));
             reserved_.push_back(0);
+            recounted_.push_back(1);
             compacted_.push_back(0);
         }
         std::vector<view_type> views;
         for (auto& t : tables_) views.push_back(t->view());
         ok_ = alloc(&views_d_, views.size()) && alloc(&first_, cfg_.max_requests) && alloc(&state_, 3);
         for (int b = 0; b < 2 && ok_; ++b) {
-            ok_ = alloc(&desc_d_[b], cfg_.max_requests) && alloc(&claims_d_[b], cfg_.max_requests) &&
+            ok_ = alloc(&desc_d_[b], cfg_.max_requests) &&
                   hipHostMalloc(reinterpret_cast<void**>(&desc_h_[b]), sizeof(desc_type) * cfg_.max_requests) ==
-                      hipSuccess &&
-                  hipHostMalloc(reinterpret_cast<void**>(&claims_h_[b]), sizeof(index_t) * cfg_.max_requests) ==
                       hipSuccess &&
                   hipHostMalloc(reinterpret_cast<void**>(&num_reqs_h_[b]), 4 * sizeof(std::uint32_t)) == hipSuccess;
         }
@@ -317,12 +308,10 @@ public:
         if (worker_.joinable()) worker_.join();
         if (stream_) (void)hipStreamSynchronize(stream_);
         for (auto& g : graphs_) if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        for (void* p : {(void*)views_d_, (void*)first_, (void*)state_, (void*)desc_d_[0], (void*)desc_d_[1],
-                        (void*)claims_d_[0], (void*)claims_d_[1]})
+        for (void* p : {(void*)views_d_, (void*)first_, (void*)state_, (void*)desc_d_[0], (void*)desc_d_[1]})
             if (p) (void)hipFree(p);
         for (int b = 0; b < 2; ++b) {
             if (desc_h_[b]) (void)hipHostFree(desc_h_[b]);
-            if (claims_h_[b]) (void)hipHostFree(claims_h_[b]);
             if (num_reqs_h_[b]) (void)hipHostFree(num_reqs_h_[b]);
         }
         if (stream_) (void)hipStreamDestroy(stream_);
@@ -393,8 +382,8 @@ public:
         std::lock_guard<std::mutex> g(mu_);
         return stats_;
     }
-    // slots of table t counted against its budget (claimed, plus the
-    // reservations of inserts still running)
+    // slots of table t counted against its budget (claimed slots as of the
+    // last recount, plus the reservations of the inserts admitted since)
     index_t reserved(index_t t) {
         std::lock_guard<std::mutex> g(mu_);
         return t < reserved_.size() ? reserved_[t] : 0;
@@ -460,16 +449,18 @@ private:
     // request. An insert that does not fit waits while steps are in flight
     // (`idle` false: their unused reservations come back when they retire),
     // and no request after it is taken until they have; with none in flight
-    // it asks for its table to be compacted (`compact` = the table) unless
-    // that was done since the table's last erase, then for its keys to be
-    // looked up (`probe`: the ones not stored are its reservation), and is
-    // refused when it does not fit even then.
-    void pick(StepList& step, desc_type* descs, bool& direct, bool idle, index_t& compact,
+    // it asks for its table's claimed slots to be recounted (`recount`),
+    // then for the table to be compacted (`compact`) unless that was done
+    // since the table's last erase, then for its keys to be looked up
+    // (`probe`: the ones not stored are its reservation), and is refused
+    // when it does not fit even then.
+    void pick(StepList& step, desc_type* descs, bool& direct, bool idle, index_t& recount, index_t& compact,
               std::shared_ptr<Request>& probe) {
         std::vector<int> mode(tables_.size(), -1);   // -1 untouched, 0 lookups, 1 mutation, 2 closed
         std::deque<std::shared_ptr<Request>> keep;
         index_t keys = 0;
         direct = false;
+        recount = ~index_t(0);
         compact = ~index_t(0);
         probe = nullptr;
         bool hold = false;   // an insert waits for the device to go idle: take nothing more
@@ -482,7 +473,8 @@ private:
             pending_.pop_front();
             const std::uint32_t t = r->d.table;
             const Op op = Op(r->d.op);
-            bool take = !direct && !hold && compact == ~index_t(0) && !probe && step.size() < cfg_.max_requests &&
+            bool take = !direct && !hold && recount == ~index_t(0) && compact == ~index_t(0) && !probe &&
+                        step.size() < cfg_.max_requests &&
                         keys < cfg_.max_step_keys && mode[t] != 2 && mode[t] != 1 &&
                         (op == Op::retrieve || mode[t] == -1);
             const bool large = latency && r->d.n - r->issued >= cfg_.direct_keys;
@@ -494,6 +486,9 @@ private:
                     // no new step starts meanwhile, so the device goes idle
                     take = false;
                     hold = true;
+                } else if (!recounted_[t]) {
+                    take = false;
+                    recount = t;
                 } else if (!compacted_[t]) {
                     take = false;
                     compact = t;
@@ -507,7 +502,7 @@ private:
             }
             if (take && op == Op::insert && r->issued == 0) {
                 reserved_[t] += want;
-                r->held = want;
+                recounted_[t] = 0;
             }
             if (!take) {
                 mode[t] = 2;   // later requests of this table wait behind it
@@ -540,6 +535,17 @@ private:
         if (latency && step.size() == 1 && step[0].keys == step[0].r->d.n) direct = true;
         pending_.swap(keep);
         waiting_.store(pending_.size(), std::memory_order_release);
+    }
+
+    // Table t's reservation becomes its claimed slots (no step in flight).
+    bool recount_table(index_t t) {
+        (void)hipGetLastError();   // drop the not-ready of earlier event polls
+        const index_t claimed = tables_[t]->claimed_slots(stream_);
+        const bool ok = hipGetLastError() == hipSuccess;
+        std::lock_guard<std::mutex> g(mu_);
+        if (ok) reserved_[t] = claimed;
+        recounted_[t] = 1;
+        return ok;
     }
 
     // Rebuilds table t without its tombstones (no step in flight): its live
@@ -616,9 +622,9 @@ private:
         (void)hipGetLastError();   // drop the not-ready of earlier event polls: report only these launches
         const index_t R = cfg_.max_requests;
         kernels::serve_prepare<desc_type><<<1, 256, 0, stream>>>(desc_h_[b], num_reqs_h_[b], R, cfg_.item_keys,
-                                                                 desc_d_[b], first_, state_, claims_d_[b]);
+                                                                 desc_d_[b], first_, state_);
         step_kernel()<<<step_grid_, block_size(), sizeof(index_t) * R, stream>>>(
-            desc_d_[b], first_, state_, R, cfg_.item_keys, views_d_, defaults::probing_length(), claims_d_[b]);
+            desc_d_[b], first_, state_, R, cfg_.item_keys, views_d_, defaults::probing_length());
         return hipGetLastError();
     }
 
@@ -641,10 +647,9 @@ private:
         index_t* first = first_;
         const index_t* first_c = first_;
         index_t* state = state_;
-        index_t* claims = claims_d_[b];
         const view_type* views = views_d_;
-        void* prep_args[] = {&staged, &staged_num, &R, &item_keys, &reqs, &first, &state, &claims};
-        void* step_args[] = {&reqs_c, &first_c, &state, &R, &item_keys, &views, &probing_length, &claims};
+        void* prep_args[] = {&staged, &staged_num, &R, &item_keys, &reqs, &first, &state};
+        void* step_args[] = {&reqs_c, &first_c, &state, &R, &item_keys, &views, &probing_length};
         hipKernelNodeParams prep{};
         prep.func = reinterpret_cast<void*>(kernels::serve_prepare<desc_type>);
         prep.gridDim = dim3(1);
@@ -669,26 +674,18 @@ private:
     }
 
     // A step of one request on its table's own bulk kernels (the kernels of
-    // one plain call), on the step stream; an insert counts its claimed
-    // slots into claims_d_[b][0].
-    hipError_t launch_direct(const desc_type& d, int b) {
+    // one plain call), on the step stream.
+    hipError_t launch_direct(const desc_type& d) {
         (void)hipGetLastError();   // drop the not-ready of earlier event polls: report only these launches
         table_type& t = *tables_[d.table];
         const index_t pl = defaults::probing_length();
         auto* st = reinterpret_cast<Status*>(d.status_out);
         using H = status_handlers::ReturnStatus;
         switch (Op(d.op)) {
-        case Op::insert: {
-            constexpr int U = detail::refill_insert_keys<unsigned(CG)>();
-            storage::kernels::fill<index_t><<<1, 64, 0, stream_>>>(claims_d_[b], 1, index_t(0));
-            if (st)
-                detail::launch_insert_refill<view_type, H, U, true>(d.keys, d.values, d.n, t.view(), pl, st, stream_,
-                                                                    nullptr, claims_d_[b]);
-            else
-                detail::launch_insert_refill<view_type, status_handlers::ReturnNothing, U, true>(
-                    d.keys, d.values, d.n, t.view(), pl, nullptr, stream_, nullptr, claims_d_[b]);
+        case Op::insert:
+            if (st) t.template insert<H>(d.keys, d.values, d.n, stream_, pl, st);
+            else t.insert(d.keys, d.values, d.n, stream_, pl);
             break;
-        }
         case Op::retrieve:
             if (st) t.template retrieve<H>(d.keys, d.n, d.values_out, stream_, pl, st);
             else t.retrieve(d.keys, d.n, d.values_out, stream_, pl);
@@ -731,36 +728,18 @@ private:
         hipEvent_t ev;
         std::uint64_t step_no;
         StepList chunks;
-        int b;                 // staging buffer (its claim counters)
-        bool inserts;          // claims copied to claims_h_[b]
         bool failed;           // launch or completion error
     };
 
-    // The oldest step is done (or failed): give back the slots its inserts
-    // did not claim, reopen compaction of tables it erased from, and
-    // complete the requests whose last chunk it ran.
+    // The oldest step is done (or failed): reopen compaction of tables it
+    // erased from, and complete the requests whose last chunk it ran.
     void retire(std::deque<InFlight>& inflight) {
         InFlight& f = inflight.front();
         {
             std::lock_guard<std::mutex> g(mu_);
-            for (std::size_t i = 0; i < f.chunks.size(); ++i) {
-                auto& c = f.chunks[i];
+            for (auto& c : f.chunks) {
                 const std::uint32_t t = c.r->d.table;
-                if (c.r->d.op == std::uint32_t(Op::insert)) {
-                    // claimed slots leave the reservation (and stay counted);
-                    // a failed step may have claimed any part of its chunk, so
-                    // a failed request keeps its whole reservation
-                    if (!f.failed) {
-                        const index_t claimed = claims_h_[f.b][i];
-                        c.r->held -= claimed < c.r->held ? claimed : c.r->held;
-                        if (c.last && !c.r->failed) {
-                            reserved_[t] -= c.r->held;
-                            c.r->held = 0;
-                        }
-                    }
-                } else if (c.r->d.op == std::uint32_t(Op::erase)) {
-                    compacted_[t] = 0;
-                }
+                if (c.r->d.op == std::uint32_t(Op::erase)) compacted_[t] = 0;
                 if (f.failed) c.r->failed = true;
                 if (!c.last) continue;
                 c.r->step = f.step_no;
@@ -781,17 +760,21 @@ private:
         while (true) {
             StepList step;
             bool direct = false;
-            index_t compact = ~index_t(0);
+            index_t recount = ~index_t(0), compact = ~index_t(0);
             std::shared_ptr<Request> probe;
             {
                 std::unique_lock<std::mutex> g(mu_);
                 if (inflight.empty()) cv_.wait(g, [&] { return stop_ || !pending_.empty(); });
                 if (stop_ && pending_.empty() && inflight.empty()) break;
                 // at most two steps in flight: descriptors are double-buffered
-                if (inflight.size() < 2) pick(step, desc_h_[b], direct, inflight.empty(), compact, probe);
+                if (inflight.size() < 2) pick(step, desc_h_[b], direct, inflight.empty(), recount, compact, probe);
                 for (auto& c : step) c.r->state.store(1);
                 if (!step.empty()) stats_.steps++;
                 if (direct) stats_.direct_steps++;
+            }
+            if (recount != ~index_t(0)) {   // no step in flight: recount, then pick again
+                (void)recount_table(recount);
+                continue;
             }
             if (compact != ~index_t(0)) {   // no step in flight: rebuild, then pick again
                 (void)compact_table(compact);
@@ -803,21 +786,16 @@ private:
             }
             if (!step.empty()) {
                 ++step_no;
-                bool inserts = false;
-                for (const auto& c : step) inserts |= c.r->d.op == std::uint32_t(Op::insert);
                 hipError_t err;
                 if (step_no == fail_step_.load()) {
                     err = hipErrorLaunchFailure;
                 } else if (direct) {
-                    err = launch_direct(desc_h_[b][0], b);
+                    err = launch_direct(desc_h_[b][0]);
                 } else {
                     num_reqs_h_[b][0] = std::uint32_t(step.size());
                     num_reqs_h_[b][1] = std::uint32_t(step_item_keys(step));
                     err = launch(b);
                 }
-                if (err == hipSuccess && inserts)
-                    err = hipMemcpyAsync(claims_h_[b], claims_d_[b], sizeof(index_t) * step.size(),
-                                         hipMemcpyDeviceToHost, stream_);
                 hipEvent_t ev = nullptr;
                 if (err == hipSuccess) err = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
                 if (err == hipSuccess) err = hipEventRecord(ev, stream_);
@@ -825,7 +803,7 @@ private:
                     (void)hipEventDestroy(ev);
                     ev = nullptr;
                 }
-                inflight.push_back(InFlight{ev, step_no, std::move(step), b, inserts, err != hipSuccess});
+                inflight.push_back(InFlight{ev, step_no, std::move(step), err != hipSuccess});
                 b ^= 1;
             }
             // retire finished steps; with a free staging buffer, go back and
@@ -854,12 +832,11 @@ private:
     bool ok_ = true;
     std::vector<std::unique_ptr<table_type>> tables_;
     std::vector<index_t> budget_, reserved_;
+    std::vector<char> recounted_;   // reserved_ is the claimed slots (no insert admitted since the recount)
     std::vector<char> compacted_;   // compacted since the table's last erase
     view_type* views_d_ = nullptr;
     desc_type* desc_d_[2] = {nullptr, nullptr};
     desc_type* desc_h_[2] = {nullptr, nullptr};
-    index_t* claims_d_[2] = {nullptr, nullptr};   // per request of a step: slots its inserts claimed
-    index_t* claims_h_[2] = {nullptr, nullptr};
     std::uint32_t* num_reqs_h_[2] = {nullptr, nullptr};
     index_t *first_ = nullptr, *state_ = nullptr;   // state: live requests, work items, keys per item
     unsigned step_grid_ = 0;

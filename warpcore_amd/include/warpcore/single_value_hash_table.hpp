@@ -582,6 +582,19 @@ public:
         return out;
     }
 
+    // slots that are not EMPTY (stored keys and tombstones; inserts never
+    // reuse a tombstone, so these are the slots taken since init)
+    WC_H index_type claimed_slots(hipStream_t stream = 0) const noexcept {
+        if (!is_initialized_) return 0;
+        index_type out = 0;
+        index_type* tmp = temp_.get();
+        (void)hipMemsetAsync(tmp, 0, sizeof(index_type), stream);
+        kernels::size<view_type><<<detail::grid_for(capacity(), 1, 8), block_size(), 0, stream>>>(tmp, view(), true);
+        (void)hipMemcpyAsync(&out, tmp, sizeof(index_type), hipMemcpyDeviceToHost, stream);
+        (void)hipStreamSynchronize(stream);
+        return out;
+    }
+
     WC_H float load_factor(hipStream_t stream = 0) const noexcept { return float(size(stream)) / float(capacity()); }
     WC_H float storage_density(hipStream_t stream = 0) const noexcept { return load_factor(stream); }
     WC_HD index_type capacity() const noexcept { return table_.capacity(); }
