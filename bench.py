@@ -57,16 +57,22 @@ def algorithmic_bytes(op: str, kb: int, vb: int) -> int:
     raise ValueError(op)
 
 
+# Revision of the probing kernels' code; a traffic figure measured on
+# another revision is not used.
+KERNEL_REVISION = "owner-2"
+
+
 def read_traffic(family: str, log2n: int = 28):
     """HBM bytes per launch of kernel `family` (insert_owner, retrieve_owner,
     ...) on this workload (2^28 keys), measured by rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, each scaled by a copy kernel of known size), if
-    profiles/svht_u64_lf09_traffic.md holds a figure for that kernel; else
-    None (a figure of another kernel is never used)."""
+    profiles/svht_u64_lf09_traffic.md holds a figure for that kernel at this
+    KERNEL_REVISION; else None (a figure of another kernel is never used)."""
     f = REPO / "profiles" / "svht_u64_lf09_traffic.md"
     if log2n != 28 or not f.exists():
         return None
-    m = re.search(rf"^traffic_bytes_per_launch\.{family}\s*=\s*([0-9.eE+]+)", f.read_text(), re.M)
+    m = re.search(rf"^traffic_bytes_per_launch\.{family}@{KERNEL_REVISION}\s*=\s*([0-9.eE+]+)", f.read_text(),
+                  re.M)
     return float(m.group(1)) if m else None
 
 
@@ -689,7 +695,7 @@ def main():
     roofline = {"bound": "hbm", "kernel": f"{fam[dom]} [SVHT u64/u64 cg 8]", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": read_traffic(fam[dom], args.log2_keys),
-                "traffic_source": "profiles/svht_u64_lf09_traffic.md",
+                "traffic_source": f"profiles/svht_u64_lf09_traffic.md @{KERNEL_REVISION}",
                 "bytes_per_op": per_op[dom], "ops_per_launch": n, "avg_kernel_ms": round(kt[dom], 4),
                 "other_kernel": {"kernel": f"{fam[other]} [SVHT u64/u64 cg 8]", "achieved": round(achieved_other, 1),
                                  "frac": round(achieved_other / HBM_PEAK_GBS, 4), "bytes_per_op": per_op[other],

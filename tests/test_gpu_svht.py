@@ -249,3 +249,57 @@ def test_bool_handler(cuda, shape, oracle):
     # invalid_key and duplicate_key leave the flag false
     np.testing.assert_array_equal(flags, oracle.return_boolean(o.insert(keys, vals)))
     assert flags.tolist() == [0, 0, 0, 0]
+
+
+@pytest.mark.parametrize("kb,vb,cg", [(64, 64, 8), (32, 32, 16), (64, 64, 4)])
+def test_overflow_filter_lookups(cuda, shape, oracle, kb, vb, cg):
+    """Lookups at high load, where most misses meet full windows: the
+    lane-owner lookups end such a miss at the first full window whose
+    overflow-filter bit is clear. Statuses and values equal the oracle's full
+    walk at load 0.97, after erasing a third and re-inserting some of it
+    (tombstones are never reused, so the re-inserts take EMPTY slots further
+    along), and once the last EMPTY slot is taken (no EMPTY slot: every miss
+    walks the whole cycle and reports probing_length_exceeded)."""
+    print(f"[svht filter] kb={kb} vb={vb} cg={cg} shape={shape}", flush=True)
+    cap = oracle.get_valid_capacity(1 << 16, cg)
+    n = int(cap * 0.97)
+    keys = unique_keys(n, kb, seed=61)
+    vals = uniform_values(n, vb, seed=62)
+    absent = unique_keys(n, kb, seed=63, exclude=keys)
+    t, o = _tables(oracle, 1 << 16, kb, vb, cg)
+    st = dev_empty(n, 32)
+    t.insert(to_dev(keys), to_dev(vals), status_out=st)
+    np.testing.assert_array_equal(from_dev(st, np.uint32), o.insert(keys, vals))
+    q = np.concatenate([keys[::2], absent])
+    got, gst = _retrieve(t, q, vb)
+    exp, ost = _oracle_retrieve(o, q, vb)
+    np.testing.assert_array_equal(gst, ost)
+    np.testing.assert_array_equal(got, exp)
+    # erase a third, re-insert some of those with new values (fewer than the
+    # EMPTY slots left)
+    er = keys[::3]
+    est = dev_empty(len(er), 32)
+    t.erase(to_dev(er), status_out=est)
+    np.testing.assert_array_equal(from_dev(est, np.uint32), o.erase(er))
+    free = cap - n
+    back = er[: free // 2]
+    bv = uniform_values(len(back), vb, seed=64)
+    st = dev_empty(len(back), 32)
+    t.insert(to_dev(back), to_dev(bv), status_out=st)
+    np.testing.assert_array_equal(from_dev(st, np.uint32), o.insert(back, bv))
+    got, gst = _retrieve(t, q, vb)
+    exp, ost = _oracle_retrieve(o, q, vb)
+    np.testing.assert_array_equal(gst, ost)
+    np.testing.assert_array_equal(got, exp)
+    assert t.peek_status() == o.status()
+    # take the last EMPTY slots: no lookup may end early any more
+    rest = unique_keys(free - len(back), kb, seed=65, exclude=np.concatenate([keys, absent]))
+    rv = uniform_values(len(rest), vb, seed=66)
+    st = dev_empty(len(rest), 32)
+    t.insert(to_dev(rest), to_dev(rv), status_out=st)
+    np.testing.assert_array_equal(from_dev(st, np.uint32), o.insert(rest, rv))
+    assert t.size() == o.size()
+    got, gst = _retrieve(t, absent[:64], vb)
+    exp, ost = _oracle_retrieve(o, absent[:64], vb)
+    np.testing.assert_array_equal(gst, ost)
+    assert np.all(ost == S_PLE)

@@ -41,6 +41,30 @@ template<class C, class = void> struct is_multi_value_core : std::false_type {};
 template<class C>
 struct is_multi_value_core<C, std::void_t<decltype(C::multi_value)>> : std::bool_constant<C::multi_value> {};
 
+// Overflow filter of a table (cores with a member `filter`: one byte per
+// probe window, a bit set for every key whose probing passed the window
+// full, the bit chosen by the key's window step). A lookup at a full window
+// without the key whose bit is clear knows the key is in no later window.
+// Cores without a filter get no-ops here.
+template<class C, class = void> struct has_overflow_filter : std::false_type {};
+template<class C>
+struct has_overflow_filter<C, std::void_t<decltype(std::declval<const C&>().filter)>> : std::true_type {};
+
+WC_HD constexpr std::uint32_t overflow_bit(index_t w, index_t step) noexcept {
+    return (std::uint32_t(1) << (step & 7)) << (8 * (w & 3));
+}
+
+// Marks window w as passed by the key of window step `step` (one
+// non-returning atomic OR).
+template<class Core>
+WC_D void mark_overflow(const Core& core, index_t w, index_t step) noexcept {
+    if constexpr (has_overflow_filter<Core>::value) {
+        if (core.filter != nullptr)
+            (void)__hip_atomic_fetch_or(core.filter + (w >> 2), overflow_bit(w, step), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Window iterator state of one key: current window, step (double hashing:
 // fixed; quadratic: growing; linear: 1) and windows left.
 template<class ProbingScheme>

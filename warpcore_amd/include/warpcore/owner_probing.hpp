@@ -107,6 +107,7 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
     cursor += 64;
 
     auto advance = [&]() {   // to the key's next window, or out of windows
+        mark_overflow(core, w, step);
         if (rnd + 1 >= budget) {
             join(Status::probing_length_exceeded());
             StatusHandler::handle(Status::probing_length_exceeded(), status_out, pos);
@@ -161,15 +162,13 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        K kj[CG];
+        // only the loaded words stay in registers while the loads are in
+        // flight; the owners' entries are read again from LDS to compare
         Probe probe[CG];
-        bool on[CG];
 #pragma unroll
         for (unsigned j = 0; j < CG; ++j) {
-            const Entry e = lds[shift + j];
-            kj[j] = e.key;
-            on[j] = e.w != kNoWindow<W>;
-            const W at = on[j] ? e.w : W(0);
+            const W ew = lds[shift + j].w;
+            const W at = ew != kNoWindow<W> ? ew : W(0);
             probe[j] = Core::load_insert_probe(slots + index_t(at) * CG + rank);
         }
         // claiming owners: CAS on the lowest untried EMPTY lane
@@ -182,9 +181,11 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
         Bits hit = 0, empty = 0;
 #pragma unroll
         for (unsigned j = 0; j < CG; ++j) {
+            const Entry e = lds[shift + j];
+            const bool on = e.w != kNoWindow<W>;
             const K tk = Core::probe_key(probe[j]);
-            const std::uint64_t hm = ballot(on[j] && tk == kj[j]);
-            const std::uint64_t em = ballot(on[j] && Core::is_empty_key(tk));
+            const std::uint64_t hm = ballot(on && tk == e.key);
+            const std::uint64_t em = ballot(on && Core::is_empty_key(tk));
             if (rank == j) {
                 hit = Bits(group_field<CG>(hm, shift));
                 empty = Bits(group_field<CG>(em, shift));
@@ -223,6 +224,16 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
 
 // Lookups of keys_in[first .. end) by the calling wavefront. A hit is
 // written by the lane holding the slot; the owner writes the status.
+//
+// Overflow filter (cores with one, see mark_overflow): a key whose window
+// is full and holds no copy of it reads the window's filter byte (next
+// round, from the MALL-sized filter instead of another window) and ends as
+// missing when its bit is clear: no key with that bit ever moved past the
+// window, so the key is in no later window. This gives the status the full
+// walk would: it is used only when the window budget covers the whole
+// cycle of windows and the wavefront has seen a window with an EMPTY slot
+// (the table is static during the call), where the walk would end at an
+// EMPTY slot with key_not_found.
 template<class Core, class StatusHandler, class W>
 WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_in, index_t first, index_t end,
                                typename Core::out_type* __restrict__ values_out, const Core& core, index_t budget,
@@ -236,7 +247,7 @@ WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_
     constexpr unsigned CG = unsigned(Core::cg_size());
     static_assert(CG >= 2 && CG <= 64 && (CG & (CG - 1)) == 0, "groups of 2..64 lanes");
     static_assert(!is_multi_value_core<Core>::value, "single-value cores only");
-    constexpr unsigned kFree = 0, kLoad = 1;
+    constexpr unsigned kFree = 0, kLoad = 1, kFilter = 2;
     const unsigned lane = lane_id();
     const unsigned rank = lane & (CG - 1);
     const unsigned shift = lane & ~(CG - 1);
@@ -245,6 +256,10 @@ WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_
     const probing_schemes::Geometry& geo = core.geometry();
     const W p = W(geo.windows);
     const Slot* __restrict__ lane_slots = core.slot_ptr() + rank;
+    const std::uint32_t* filter = nullptr;
+    if constexpr (has_overflow_filter<Core>::value)
+        if (budget >= index_t(geo.windows)) filter = core.filter;
+    bool seen_empty = false;   // some window of the table has an EMPTY slot
     if (end <= first) return;
     if (!core.ready()) {
         for (index_t i = first + lane; i < end; i += 64)
@@ -302,51 +317,66 @@ WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        K kj[CG];
-        std::uint32_t pj[CG];
+        // only the loaded slots stay in registers while the loads are in
+        // flight; the owners' entries are read again from LDS to compare
         Slot s[CG];
-        bool on[CG];
+#pragma unroll
+        for (unsigned j = 0; j < CG; ++j) {
+            const W ew = lds[shift + j].w;
+            const W at = ew != kNoWindow<W> ? ew : W(0);
+            s[j] = load_slot(lane_slots + index_t(at) * CG);
+        }
+        std::uint32_t fword = 0;
+        if (phase == kFilter) fword = filter[w >> 2];
+        __builtin_amdgcn_wave_barrier();
+        bool hit = false, empty = false;
+        std::uint64_t any_empty = 0;
 #pragma unroll
         for (unsigned j = 0; j < CG; ++j) {
             const Entry e = lds[shift + j];
-            kj[j] = e.key;
-            pj[j] = e.pos;
-            on[j] = e.w != kNoWindow<W>;
-            const W at = on[j] ? e.w : W(0);
-            s[j] = load_slot(lane_slots + index_t(at) * CG);
-        }
-        __builtin_amdgcn_wave_barrier();
-        bool hit = false, empty = false;
-#pragma unroll
-        for (unsigned j = 0; j < CG; ++j) {
+            const bool on = e.w != kNoWindow<W>;
             const K tk = Core::key_of(s[j]);
-            const bool h = on[j] && tk == kj[j];
-            if (h) values_out[first + pj[j]] = core.found(Core::value_of(s[j]));   // the slot's lane writes the hit
+            const bool h = on && tk == e.key;
+            if (h) values_out[first + e.pos] = core.found(Core::value_of(s[j]));   // the slot's lane writes the hit
             const std::uint64_t hm = ballot(h);
-            const std::uint64_t em = ballot(on[j] && Core::is_empty_key(tk));
+            const std::uint64_t em = ballot(on && Core::is_empty_key(tk));
+            any_empty |= em;
             if (rank == j) {
                 hit = group_field<CG>(hm, shift) != 0;
                 empty = group_field<CG>(em, shift) != 0;
             }
         }
+        seen_empty = seen_empty || any_empty != 0;
+        auto miss = [&]() {
+            join(miss_status);
+            StatusHandler::handle(miss_status, status_out, pos);
+            if (!miss_status.has_any()) values_out[pos] = miss_value;
+            phase = kFree;
+        };
+        auto next_window = [&]() {
+            const W nw = w + step;
+            w = nw >= p ? W(nw - p) : nw;
+            rnd += 1;
+            phase = kLoad;
+        };
         if (phase == kLoad) {
             if (hit) {
                 StatusHandler::handle(Status::none(), status_out, pos);
                 phase = kFree;
             } else if (empty) {
-                join(miss_status);
-                StatusHandler::handle(miss_status, status_out, pos);
-                if (!miss_status.has_any()) values_out[pos] = miss_value;
-                phase = kFree;
+                miss();
             } else if (rnd + 1 >= budget) {
                 join(Status::probing_length_exceeded());
                 StatusHandler::handle(Status::probing_length_exceeded(), status_out, pos);
                 phase = kFree;
+            } else if (filter != nullptr && seen_empty) {
+                phase = kFilter;   // read the window's filter byte next round
             } else {
-                const W nw = w + step;
-                w = nw >= p ? W(nw - p) : nw;
-                rnd += 1;
+                next_window();
             }
+        } else if (phase == kFilter) {
+            if ((fword & overflow_bit(w, step)) == 0) miss();
+            else next_window();
         }
     }
 }

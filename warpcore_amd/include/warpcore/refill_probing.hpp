@@ -552,6 +552,7 @@ WC_D void insert_refill_range(const typename Core::key_type* __restrict__ keys_i
             if (rank == 0 && mine(last)) StatusHandler::handle(ple, status_out, first + pos[u]);
             const std::uint64_t move = adv & ~last;
             const W nw = w[u] + step[u];
+            if (rank == 0 && mine(move)) mark_overflow(core, w[u], step[u]);
             if (mine(move)) {
                 w[u] = nw >= p ? W(nw - p) : nw;
                 rnd[u] += 1;

@@ -57,6 +57,9 @@ struct SingleValueView {
     probing_schemes::Geometry geo;
     Key seed;
     bool initialized;
+    // overflow filter (one byte per window, see mark_overflow), or null:
+    // kept by every insert path of the table, read by the lane-owner lookups
+    std::uint32_t* filter;
 
     WC_HD static constexpr index_type cg_size() noexcept { return ProbingScheme::cg_size(); }
     WC_HD static constexpr bool is_empty_key(Key k) noexcept { return k == EmptyKey; }
@@ -151,6 +154,14 @@ struct SingleValueView {
             return ~index_type(0);
         }
         ProbingScheme iter(geo, probing_length, group);
+        index_type step = 0;   // the key's window step, for the overflow filter
+        if constexpr (fixed_step_scheme<ProbingScheme>::value) {
+            if (filter != nullptr) {
+                WindowCursor<ProbingScheme> c;
+                c.begin(key_in, seed, geo);
+                step = c.step;
+            }
+        }
         for (index_type i = iter.begin(key_in, seed); i != iter.end(); i = iter.next()) {
             const Slot s = load_slot(slots + i);
             const Key table_key = s.key;
@@ -205,6 +216,9 @@ struct SingleValueView {
                 }
                 empty_mask ^= std::uint64_t(1) << leader;
             }
+            // the key moves on past a window without room
+            if constexpr (fixed_step_scheme<ProbingScheme>::value)
+                if (group.thread_rank() == 0) mark_overflow(*this, i / cg_size(), step);
         }
         join(Status::probing_length_exceeded());
         status_out = Status::probing_length_exceeded();
@@ -451,23 +465,29 @@ public:
           is_copy_(false),
           geo_(probing_schemes::Geometry::make(table_.capacity(), cg_size())) {
         if (hipMalloc(&status_, sizeof(status_type)) != hipSuccess) status_ = nullptr;
+        if constexpr (with_filter) {
+            if (table_.capacity() != 0 && hipMalloc(&filter_, filter_bytes()) != hipSuccess) filter_ = nullptr;
+        }
         assign_status(table_.status() + temp_.status());
         if (!no_init) init();
     }
 
     WC_HD SingleValueHashTable(const SingleValueHashTable& o) noexcept
         : status_(o.status_), table_(o.table_), temp_(o.temp_), seed_(o.seed_),
-          is_initialized_(o.is_initialized_), is_copy_(true), geo_(o.geo_) {}
+          is_initialized_(o.is_initialized_), is_copy_(true), geo_(o.geo_), filter_(o.filter_),
+          filter_valid_(o.filter_valid_) {}
 
     WC_H SingleValueHashTable(SingleValueHashTable&& o) noexcept
         : status_(o.status_), table_(std::move(o.table_)), temp_(std::move(o.temp_)), seed_(o.seed_),
-          is_initialized_(o.is_initialized_), is_copy_(o.is_copy_), geo_(o.geo_) {
+          is_initialized_(o.is_initialized_), is_copy_(o.is_copy_), geo_(o.geo_), filter_(o.filter_),
+          filter_valid_(o.filter_valid_) {
         o.is_copy_ = true;
     }
 
     WC_HD ~SingleValueHashTable() noexcept {
 #if !defined(__HIP_DEVICE_COMPILE__)
         if (!is_copy_ && status_ != nullptr) (void)hipFree(status_);
+        if (!is_copy_ && filter_ != nullptr) (void)hipFree(filter_);
 #endif
     }
 
@@ -477,6 +497,11 @@ public:
         seed_ = seed;
         if (!table_.status().has_not_initialized() && !temp_.status().has_not_initialized()) {
             table_.init_keys(empty_key(), stream);
+            if (filter_ != nullptr)
+                storage::kernels::fill<std::uint32_t>
+                    <<<detail::grid_for(filter_bytes() / 4, 1, 8), block_size(), 0, stream>>>(
+                        filter_, filter_bytes() / 4, 0u);
+            filter_valid_ = filter_ != nullptr;
             assign_status(table_.status() + temp_.status(), stream);
             is_initialized_ = true;
         }
@@ -599,7 +624,7 @@ public:
     WC_H float storage_density(hipStream_t stream = 0) const noexcept { return load_factor(stream); }
     WC_HD index_type capacity() const noexcept { return table_.capacity(); }
     WC_H index_type bytes_total() const noexcept {
-        return table_.bytes_total() + temp_.bytes_total() + sizeof(status_type);
+        return table_.bytes_total() + temp_.bytes_total() + sizeof(status_type) + (filter_ ? filter_bytes() : 0);
     }
     WC_HD bool is_initialized() const noexcept { return is_initialized_; }
 
@@ -634,7 +659,7 @@ public:
         if constexpr (soa_storage) {
             return view_type{table_.keys(), table_.values(), status_, geo_, seed_, is_initialized_};
         } else {
-            return view_type{table_.data(), status_, geo_, seed_, is_initialized_};
+            return view_type{table_.data(), status_, geo_, seed_, is_initialized_, filter_valid_ ? filter_ : nullptr};
         }
     }
 
@@ -646,6 +671,7 @@ private:
                          index_type probing_length, typename StatusHandler::base_type* status_out) noexcept {
         if constexpr (detail::LaneWindow<slot_type, cg_size()>::usable) {
             if (capacity() * sizeof(slot_type) <= detail::lane_insert_table_bytes()) {
+                filter_valid_ = false;   // these kernels do not keep the overflow filter
                 kernels::insert_lanes<view_type, StatusHandler, detail::keys_per_lane(), true>
                     <<<detail::grid_for_lanes(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, view(),
                                                                                  probing_length, status_out);
@@ -659,12 +685,14 @@ private:
                 return;
             }
             if (detail::lockstep_inserts()) {
+                filter_valid_ = false;
                 detail::launch_insert_lockstep<view_type, StatusHandler,
                                                detail::lockstep_insert_keys<unsigned(cg_size())>(), true>(
                     keys_in, values_in, num_in, view(), probing_length, status_out, stream);
                 return;
             }
         }
+        filter_valid_ = false;
         kernels::insert_batched<view_type, StatusHandler, detail::keys_per_group(), true>
             <<<detail::grid_for_batched(num_in, cg_size()), block_size(), 0, stream>>>(
                 keys_in, values_in, num_in, view(), probing_length, status_out);
@@ -714,6 +742,14 @@ private:
         if (status_ != nullptr) status_->atomic_join(status);
     }
 
+    // overflow filter: AoS tables probed with a fixed step keep one byte
+    // per probe window (every insert path the table uses maintains it; a
+    // lookup that meets a full window without its key and with its bit clear
+    // ends there, see owner_probing.hpp)
+    static constexpr bool with_filter =
+        !soa_storage && detail::fixed_step_scheme<ProbingScheme>::value;
+    WC_HD index_type filter_bytes() const noexcept { return SDIV(geo_.windows, index_type(4)) * 4; }
+
     status_type* status_;
     TableStorage table_;
     temp_type temp_;
@@ -721,6 +757,10 @@ private:
     bool is_initialized_;
     bool is_copy_;
     probing_schemes::Geometry geo_;
+    std::uint32_t* filter_ = nullptr;
+    // the filter holds every insert since init (cleared when an insert ran
+    // on kernels that do not maintain it; lookups then ignore it)
+    mutable bool filter_valid_ = false;
 
     template<class K_, class V_, K_ E_, K_ T_, class P_, class S_, index_t B_> friend class CountingHashTable;
     template<class K_, class V_, K_ E_, K_ T_, class VS_, class P_> friend class BucketListHashTable;
