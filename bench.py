@@ -229,7 +229,9 @@ def run_batched(g: Gpu, num_tables: int = 65536, steps: int = 3, warmup: int = 1
     """65,536 SingleValueHashTable<u32,u32> of capacity log-uniform in
     2^6..2^16 slots, each filled to a load factor uniform in [0.5, 0.9]:
     insert of every table's keys, then retrieve of them, one persistent
-    launch each. Beside it, plain per-table launches on a sample."""
+    launch each, after the autotuner has chosen (lanes, window) per
+    (capacity, fill) class and the launch. Beside it, plain per-table
+    launches on a sample."""
     import numpy as np
     torch = g.torch
     from warpcore_amd.tables import SingleValueHashTable, TableBatch
@@ -245,9 +247,18 @@ def run_batched(g: Gpu, num_tables: int = 65536, steps: int = 3, warmup: int = 1
     offs = torch.from_numpy(off.view(np.int64)).to("cuda")
     batch = TableBatch(caps, 32, 32, cg_size=16)
     g.stream.synchronize()
-    launch, tuned_ms = None, None
+    launch, tuned_ms, classes, totals = None, None, None, None
     if tune:
+        t0 = time.perf_counter()
         launch, tuned_ms = batch.tune(keys, values, offs, keys, offs, out, stream=g.stream)
+        tune_s = time.perf_counter() - t0
+        rows, totals = batch.tune_report()
+        classes = [{"capacity": f"2^{2 * r['capacity_band']}..2^{2 * r['capacity_band'] + 2}",
+                    "fill": ["<0.6", "0.6-0.7", "0.7-0.8", ">=0.8"][r["fill_band"]],
+                    "tables": r["tables"], "keys": r["keys"], "chosen": list(r["chosen_shape"]),
+                    "fixed_ms": round(r["fixed_ns"] / 1e6, 4), "chosen_ms": round(r["chosen_ns"] / 1e6, 4),
+                    "gain": round(r["fixed_ns"] / max(r["chosen_ns"], 1), 3),
+                    "slots_ratio": round(r["chosen_slots"] / max(r["fixed_slots"], 1), 3)} for r in rows]
     for _ in range(warmup):
         batch.init(stream=g.stream)
         batch.insert(keys, values, offs, stream=g.stream)
@@ -282,9 +293,14 @@ def run_batched(g: Gpu, num_tables: int = 65536, steps: int = 3, warmup: int = 1
     plain_ms = e0.elapsed_time(e1)
     res = {"tables": num_tables, "keys": n, "total_slots": batch.total_slots(),
            "gops": round(2 * n / (ins + ret) / 1e6, 3), "insert_gops": round(n / ins / 1e6, 3),
-           "retrieve_gops": round(n / ret / 1e6, 3), "launch": list(launch) if launch else None,
-           "results_ok": ok, "plain_sample_tables": sample_tables,
+           "retrieve_gops": round(n / ret / 1e6, 3), "results_ok": ok, "plain_sample_tables": sample_tables,
            "plain_per_table_gops": round(2 * sample_keys / plain_ms / 1e6, 3)}
+    if tune:
+        res["tuner"] = {"launch": {"item_keys": launch[0], "blocks_per_cu": launch[1], "block_size": launch[2]},
+                        "fixed": "(16,16) every table, launch 2048 keys/item, 8 blocks/CU of 256",
+                        "fixed_gops": round(2 * n / totals[0] / 1e6, 3),
+                        "tuned_gops_in_tune": round(2 * n / totals[1] / 1e6, 3),
+                        "tune_seconds": round(tune_s, 2), "classes": classes}
     del batch, tables, keys, values, out, offs
     torch.cuda.empty_cache()
     return res

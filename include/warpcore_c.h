@@ -520,20 +520,44 @@ WC_API uint32_t wc_bucketlist_u64u64_pop_status(void* table, void* stream);
 WC_API uint32_t wc_bucketlist_u64u64_cg_size(void* table);
 
 /* ------------------------------------------------------------------------
- * TableBatch<K, V, CG>: many small SingleValueHashTables in one arena (no
+ * TableBatch<K, V>: many small SingleValueHashTables in one arena (no
  * reference counterpart; the reference builds one table per object with at
- * least 131101*cg slots). Table t gets the smallest prime >= SDIV(min, cg)
- * windows of cg slots (cg 4, 8 or 16; 0 = 16).
- *   create       min_capacities is a HOST array of num_tables slot counts
+ * least 131101*cg slots). Each table has a configuration (lanes, window):
+ * groups of `lanes` lanes probe windows of `window` slots; table t gets the
+ * smallest prime >= max(2, SDIV(min, window)) windows of `window` slots.
+ * Configurations, by index: u32u32 (4,4) (4,8) (8,8) (8,16) (16,16);
+ * u64u64 (4,4) (8,8) (16,16).
+ *   create       min_capacities is a HOST array of num_tables slot counts;
+ *                every table starts at configuration (cg_size, cg_size)
+ *                (cg_size 4, 8 or 16; 0 = 16)
  *   insert / retrieve  keys of all tables concatenated; offsets is a DEVICE
  *                array of num_tables + 1 (table t owns [offsets[t],
  *                offsets[t+1])); one persistent work-stealing launch; per
  *                key results and statuses as SingleValueHashTable's
- *   set_launch / get_launch  keys per lockstep batch (4 or 8), keys per
- *                work item, blocks per CU; tune times the candidates on the
- *                given insert + retrieve workload, keeps the fastest and
- *                leaves the batch empty (launch_out: 3 values, ms_out: best
- *                insert + retrieve time)
+ *   set_launch / get_launch  keys per work item, blocks per CU, threads per
+ *                block (64..512, a multiple of 64)
+ *   tune         autotuner: tables are put in classes by requested capacity
+ *                (bands [4^k, 4^(k+1))) and fill (keys / requested capacity:
+ *                < 0.6, < 0.7, < 0.8, >= 0.8); every configuration is timed
+ *                on every class alone (insert + retrieve of the given
+ *                workload restricted to the class), each class keeps its
+ *                fastest configuration within 10% of its fewest slots, then
+ *                the launch candidates are timed on the whole workload.
+ *                Leaves the batch empty with the chosen configurations and
+ *                launch. launch_out: 3 values as get_launch; ms_out: tuned
+ *                insert + retrieve time of the whole workload
+ *   tune_report  after a tune: rows of 10 values per class (class id =
+ *                capacity band * 4 + fill band, tables, keys, queries,
+ *                fixed configuration, chosen configuration, fixed ns, chosen
+ *                ns, fixed slots, chosen slots), at most max_rows written;
+ *                totals[2] = whole workload ms at the creation configuration
+ *                with the default launch, and tuned. Returns the number of
+ *                classes (0 before a tune)
+ *   set_configs  per-table configuration indices (HOST array of num_tables
+ *                bytes); lays the tables out again and empties the batch
+ *   table_config / num_configs / config_shape  a table's configuration
+ *                index; the number of configurations; (lanes, window) of one
+ *   cg_size      lanes of the creation configuration
  *   copy_status_words  per-table status words into a device buffer
  * ---------------------------------------------------------------------- */
 /* TableBatch<uint32_t, uint32_t> */
@@ -542,9 +566,14 @@ WC_API void wc_batch_u32u32_destroy(void* batch);
 WC_API int wc_batch_u32u32_init(void* batch, void* stream);
 WC_API int wc_batch_u32u32_insert(void* batch, const uint32_t* keys_in, const uint32_t* values_in, const uint64_t* offsets, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_batch_u32u32_retrieve(void* batch, const uint32_t* keys_in, const uint64_t* offsets, uint64_t num_in, uint32_t* values_out, void* stream, uint64_t probing_length, uint32_t* status_out);
-WC_API int wc_batch_u32u32_set_launch(void* batch, uint32_t keys_per_batch, uint64_t item_keys, uint32_t blocks_per_cu);
+WC_API int wc_batch_u32u32_set_launch(void* batch, uint64_t item_keys, uint32_t blocks_per_cu, uint32_t block_size);
 WC_API int wc_batch_u32u32_get_launch(void* batch, uint64_t* launch_out);
 WC_API int wc_batch_u32u32_tune(void* batch, const uint32_t* keys_in, const uint32_t* values_in, const uint64_t* offsets, uint64_t num_in, const uint32_t* queries, const uint64_t* query_offsets, uint64_t num_queries, uint32_t* values_out, void* stream, uint64_t* launch_out, double* ms_out);
+WC_API int wc_batch_u32u32_tune_report(void* batch, uint64_t* rows, uint64_t max_rows, double* totals);
+WC_API int wc_batch_u32u32_set_configs(void* batch, const uint8_t* configs, uint64_t num_tables);
+WC_API int wc_batch_u32u32_table_config(void* batch, uint64_t table);
+WC_API int wc_batch_u32u32_num_configs(void);
+WC_API int wc_batch_u32u32_config_shape(uint32_t config, uint32_t* lanes_window);
 WC_API uint64_t wc_batch_u32u32_num_tables(void* batch);
 WC_API uint64_t wc_batch_u32u32_capacity(void* batch, uint64_t table);
 WC_API uint64_t wc_batch_u32u32_total_slots(void* batch);
@@ -558,9 +587,14 @@ WC_API void wc_batch_u64u64_destroy(void* batch);
 WC_API int wc_batch_u64u64_init(void* batch, void* stream);
 WC_API int wc_batch_u64u64_insert(void* batch, const uint64_t* keys_in, const uint64_t* values_in, const uint64_t* offsets, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_batch_u64u64_retrieve(void* batch, const uint64_t* keys_in, const uint64_t* offsets, uint64_t num_in, uint64_t* values_out, void* stream, uint64_t probing_length, uint32_t* status_out);
-WC_API int wc_batch_u64u64_set_launch(void* batch, uint32_t keys_per_batch, uint64_t item_keys, uint32_t blocks_per_cu);
+WC_API int wc_batch_u64u64_set_launch(void* batch, uint64_t item_keys, uint32_t blocks_per_cu, uint32_t block_size);
 WC_API int wc_batch_u64u64_get_launch(void* batch, uint64_t* launch_out);
 WC_API int wc_batch_u64u64_tune(void* batch, const uint64_t* keys_in, const uint64_t* values_in, const uint64_t* offsets, uint64_t num_in, const uint64_t* queries, const uint64_t* query_offsets, uint64_t num_queries, uint64_t* values_out, void* stream, uint64_t* launch_out, double* ms_out);
+WC_API int wc_batch_u64u64_tune_report(void* batch, uint64_t* rows, uint64_t max_rows, double* totals);
+WC_API int wc_batch_u64u64_set_configs(void* batch, const uint8_t* configs, uint64_t num_tables);
+WC_API int wc_batch_u64u64_table_config(void* batch, uint64_t table);
+WC_API int wc_batch_u64u64_num_configs(void);
+WC_API int wc_batch_u64u64_config_shape(uint32_t config, uint32_t* lanes_window);
 WC_API uint64_t wc_batch_u64u64_num_tables(void* batch);
 WC_API uint64_t wc_batch_u64u64_capacity(void* batch, uint64_t table);
 WC_API uint64_t wc_batch_u64u64_total_slots(void* batch);

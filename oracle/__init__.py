@@ -66,6 +66,7 @@ _SIGS = {
     "wco_get_valid_capacity": (_u64, [_u64, _u64]),
     "wco_probe_sequence": (_u64, [_i, _i, _u64, _u64, _u64, _u64, _u64, _vp, _u64]),
     "wco_svht_create": (_vp, [_i, _i, _u64, _u64, _i, _u64]), "wco_svht_destroy": (None, [_vp]),
+    "wco_svht_create_sized": (_vp, [_i, _i, _u64, _u64, _i, _u64]), "wco_batch_windows": (_u64, [_u64, _u64]),
     "wco_svht_capacity": (_u64, [_vp]), "wco_svht_size": (_u64, [_vp]), "wco_svht_status": (_u32, [_vp]),
     "wco_svht_clear_status": (None, [_vp]),
     "wco_svht_insert": (None, [_vp, _vp, _vp, _u64, _u64, _vp]),
@@ -169,6 +170,12 @@ def get_valid_capacity(min_capacity: int, cg: int) -> int:
     return lib().wco_get_valid_capacity(min_capacity, cg)
 
 
+def batch_capacity(min_capacity: int, window: int) -> int:
+    """Slots of a TableBatch table: the smallest prime >= max(2,
+    SDIV(min_capacity, window)) windows of `window` slots."""
+    return lib().wco_batch_windows(min_capacity, window) * window
+
+
 def primes() -> np.ndarray:
     n = lib().wco_num_primes()
     out = np.empty(n, np.uint64)
@@ -198,9 +205,15 @@ class _Handle:
 class SVHT(_Handle):
     _prefix = "wco_svht_"
 
-    def __init__(self, min_capacity, key_bits=32, value_bits=32, cg=8, scheme=AL_DOUBLE, seed=DEFAULT_SEED):
+    def __init__(self, min_capacity, key_bits=32, value_bits=32, cg=8, scheme=AL_DOUBLE, seed=DEFAULT_SEED,
+                 exact_capacity=None):
+        """exact_capacity: exactly that many slots (a multiple of cg), the
+        geometry of a TableBatch table, instead of the reference's rule."""
         self.kb, self.vb = key_bits, value_bits
-        self.h = self._f("create")(key_bits, value_bits, min_capacity, cg, scheme, seed)
+        if exact_capacity is None:
+            self.h = self._f("create")(key_bits, value_bits, min_capacity, cg, scheme, seed)
+        else:
+            self.h = self._f("create_sized")(key_bits, value_bits, exact_capacity, cg, scheme, seed)
 
     def insert(self, keys, values, pl=DEFAULT_PL):
         keys = np.ascontiguousarray(keys, _dt(self.kb)); values = np.ascontiguousarray(values, _dt(self.vb))

@@ -140,6 +140,25 @@ void* wco_svht_create(int kb, int vb, u64 min_cap, u64 cg, int scheme, u64 seed)
     }
     return o;
 }
+// A table of exactly `capacity` slots (a multiple of cg): the geometry of
+// a TableBatch table, which has no reference counterpart.
+void* wco_svht_create_sized(int kb, int vb, u64 capacity, u64 cg, int scheme, u64 seed) {
+    Obj* o = (Obj*)wco_svht_create(kb, vb, cg, cg, scheme, seed);
+    with<SVHT>(o, [&](auto& t) {
+        t.cap = capacity;
+        t.init_ok = capacity != 0 && capacity % cg == 0;
+        t.status = t.init_ok ? S_NONE : (S_INVALID_CONFIG | S_NOT_INIT);
+        t.keys.assign(capacity, t.EMPTY);
+        t.vals.assign(capacity, 0);
+        return 0;
+    });
+    return o;
+}
+// TableBatch's small-table rule: the smallest prime >= max(2, SDIV(min, window)) windows
+u64 wco_batch_windows(u64 min_cap, u64 window) {
+    const u64 w = (min_cap + window - 1) / window;
+    return next_prime(w < 2 ? 2 : w);
+}
 void wco_svht_destroy(void* h) {
     Obj* o = (Obj*)h;
     with<SVHT>(o, [](auto& t) { delete &t; return 0; });

@@ -32,34 +32,61 @@ def _workload(num_tables, kb, seed, lo=6, hi=14, lf=(0.5, 0.9), absent_frac=0.5)
     return caps, keys, vals, queries, off, qoff
 
 
-@pytest.mark.parametrize("kb,cg", [(32, 16), (32, 8), (64, 8), (32, 4)])
-def test_table_batch_matches_oracle_per_table(cuda, oracle, kb, cg):
+def _configs(kb):
     from warpcore_amd.tables import TableBatch
-    print(f"[batch] kb={kb} cg={cg}", flush=True)
-    caps, keys, vals, queries, off, qoff = _workload(600, kb, seed=3 + cg)
-    b = TableBatch(caps, kb, kb, cg_size=cg)
+    b = TableBatch(np.array([64], np.uint64), kb, kb)
+    return b.configs()
+
+
+def _check_against_oracle(oracle, b, kb, caps, keys, vals, queries, qoff, st, got, gst, words, every=7):
+    """Per sampled table: capacity by the small-table rule, and the oracle
+    at exactly that geometry (capacity, window) fed the same keys."""
+    shapes = b.configs()
+    for t in range(0, len(caps), every):
+        window = shapes[b.table_config(t)][1]
+        assert b.capacity(t) == oracle.batch_capacity(int(caps[t]), window)
+        o = oracle.SVHT(0, kb, kb, cg=window, exact_capacity=b.capacity(t))
+        np.testing.assert_array_equal(st[t], o.insert(keys[t], vals[t]), err_msg=f"table {t}")
+        ov, ost = o.retrieve(queries[t])
+        sl = slice(int(qoff[t]), int(qoff[t + 1]))
+        np.testing.assert_array_equal(gst[sl], ost, err_msg=f"table {t}")
+        hit = ost == S_NONE
+        np.testing.assert_array_equal(got[sl][hit], ov[hit])
+        assert np.all(got[sl][~hit] == np.array(0xAB, dtype(kb)))   # misses leave the output alone
+        assert words[t] == o.status(), f"table {t}"
+
+
+# every configuration of each key width (index into TableBatch.configs()),
+# and "mixed": a random configuration per table
+CONFIG_CASES = [(32, c) for c in range(5)] + [(64, c) for c in range(3)] + [(32, "mixed"), (64, "mixed")]
+
+
+@pytest.mark.parametrize("kb,config", CONFIG_CASES)
+def test_table_batch_matches_oracle_per_table(cuda, oracle, kb, config):
+    from warpcore_amd.tables import TableBatch
+    print(f"[batch] kb={kb} config={config}", flush=True)
+    caps, keys, vals, queries, off, qoff = _workload(600, kb, seed=3 + (17 if config == "mixed" else config))
+    b = TableBatch(caps, kb, kb)
+    n_cfg = len(b.configs())
+    if config == "mixed":
+        b.set_configs(np.random.default_rng(5).integers(0, n_cfg, len(caps)))
+    else:
+        b.set_configs(np.full(len(caps), config))
     assert b.num_tables() == len(caps)
     for t in (0, 17, 599):
-        assert b.capacity(t) >= caps[t] and b.capacity(t) % cg == 0
+        assert b.capacity(t) >= caps[t] and b.capacity(t) % b.window(t) == 0
     K, V, Q = (to_dev(np.concatenate(x)) for x in (keys, vals, queries))
     st = dev_empty(K.numel(), 32, 0x77)
     b.insert(K, V, to_dev(off), status_out=st)
-    assert np.all(from_dev(st, np.uint32) == S_NONE)
+    ist = from_dev(st, np.uint32)
+    assert np.all(ist == S_NONE)
     out = dev_empty(Q.numel(), kb, 0xAB)
     qst = dev_empty(Q.numel(), 32, 0x77)
     b.retrieve(Q, to_dev(qoff), out, status_out=qst)
     got, gst = from_dev(out, dtype(kb)), from_dev(qst, np.uint32)
     words = from_dev(b.status_words(), np.uint32)
-    for t in range(0, len(caps), 7):
-        o = oracle.SVHT(int(caps[t]), kb, kb, cg=cg)
-        assert np.all(o.insert(keys[t], vals[t]) == S_NONE)
-        ov, ost = o.retrieve(queries[t])
-        sl = slice(int(qoff[t]), int(qoff[t + 1]))
-        np.testing.assert_array_equal(gst[sl], ost)
-        hit = ost == S_NONE
-        np.testing.assert_array_equal(got[sl][hit], ov[hit])
-        assert np.all(got[sl][~hit] == np.array(0xAB, dtype(kb)))   # misses leave the output alone
-        assert words[t] == o.status()
+    per_table = [ist[int(off[t]):int(off[t + 1])] for t in range(len(caps))]
+    _check_against_oracle(oracle, b, kb, caps, keys, vals, queries, qoff, per_table, got, gst, words)
 
 
 def test_table_batch_matches_plain_launches(cuda):
@@ -90,14 +117,20 @@ def test_table_batch_matches_plain_launches(cuda):
         np.testing.assert_array_equal(got[q], from_dev(pout, np.uint32))
 
 
-def test_table_batch_full_tables_and_invalid_keys(cuda):
+@pytest.mark.parametrize("kb,shape", [(32, (16, 16)), (32, (4, 8)), (32, (8, 16)), (64, (4, 4))])
+def test_table_batch_full_tables_and_invalid_keys(cuda, oracle, kb, shape):
+    """Tables filled past capacity (every key beyond it reports
+    probing_length_exceeded), EMPTY and TOMBSTONE keys, per-table status
+    words; the same statuses as the oracle at the table's geometry."""
     from warpcore_amd.tables import TableBatch
     caps = np.array([64, 64, 200], np.uint64)
-    b = TableBatch(caps, 32, 32, cg_size=16)
+    b = TableBatch(caps, kb, kb)
+    b.set_configs(np.full(3, b.configs().index(shape)))
     c0 = b.capacity(0)
-    k0 = unique_keys(c0 + 20, 32, seed=1)          # 20 more keys than slots
-    k1 = np.array([5, 0, 6, 0xFFFFFFFF], np.uint32)  # EMPTY and TOMBSTONE are invalid
-    k2 = unique_keys(100, 32, seed=2)
+    k0 = unique_keys(c0 + 20, kb, seed=1)          # 20 more keys than slots
+    tomb = (1 << kb) - 1
+    k1 = np.array([5, 0, 6, tomb], dtype(kb))        # EMPTY and TOMBSTONE are invalid
+    k2 = unique_keys(100, kb, seed=2)
     keys = [k0, k1, k2]
     off = np.concatenate([[0], np.cumsum([len(k) for k in keys])]).astype(np.uint64)
     K = to_dev(np.concatenate(keys))
@@ -109,37 +142,68 @@ def test_table_batch_full_tables_and_invalid_keys(cuda):
     assert list(s1) == [S_NONE, S_INVALID_KEY, S_NONE, S_INVALID_KEY]
     words = from_dev(b.status_words(), np.uint32)
     assert words[0] == S_PLE and words[1] == S_INVALID_KEY and words[2] == S_NONE
+    # the oracle (sequential) fills the same slots count; which keys are
+    # refused depends on the interleaving, so compare counts for table 0
+    for t, k in enumerate(keys):
+        o = oracle.SVHT(0, kb, kb, cg=shape[1], exact_capacity=b.capacity(t))
+        ost = o.insert(k, k)
+        got = s[int(off[t]):int(off[t + 1])]
+        if t == 0:
+            assert np.sum(ost == S_PLE) == 20 and np.sum(ost == S_NONE) == c0
+        else:
+            np.testing.assert_array_equal(got, ost)
+        assert words[t] == o.status()
     b.init()
     assert np.all(from_dev(b.status_words(), np.uint32) == 0)
 
 
-def test_table_batch_tuner_keeps_results(cuda):
+@pytest.mark.parametrize("kb", [32, 64])
+def test_table_batch_tuner(cuda, oracle, kb):
+    """The autotuner: one row per (capacity, fill) class present, chosen
+    configurations within the slot rule, a launch among the candidates;
+    afterwards the batch is empty and its results match the oracle at every
+    table's chosen geometry, under the tuned launch and under others."""
     from warpcore_amd.tables import TableBatch
-    caps, keys, vals, queries, off, qoff = _workload(300, 32, seed=21)
-    b = TableBatch(caps, 32, 32, cg_size=16)
+    caps, keys, vals, queries, off, qoff = _workload(400, kb, seed=21, lo=6, hi=14)
+    b = TableBatch(caps, kb, kb, cg_size=16)
     K, V, Q = (to_dev(np.concatenate(x)) for x in (keys, vals, queries))
     O, QO = to_dev(off), to_dev(qoff)
-    out = dev_empty(Q.numel(), 32)
+    out = dev_empty(Q.numel(), kb)
     launch, ms = b.tune(K, V, O, Q, QO, out)
-    assert launch[0] in (4, 8) and launch[1] in (32, 64, 128, 256) and launch[2] in (4, 8, 16)
+    assert launch[0] in (512, 2048, 8192) and launch[2] in (128, 256, 512)
+    assert launch[1] * launch[2] // 64 in (16, 32)
     assert ms > 0 and b.get_launch() == launch
-    for cfg in [launch, (4, 32, 4), (8, 256, 16)]:
+    rows, (fixed_ms, tuned_ms) = b.tune_report()
+    assert fixed_ms > 0 and tuned_ms == pytest.approx(ms)
+    fills = [len(k) / float(c) for k, c in zip(keys, caps)]
+    expect = sorted({(int(c).bit_length() - 1) // 2 * 4 + (0 if f < 0.6 else 1 if f < 0.7 else 2 if f < 0.8 else 3)
+                     for c, f in zip(caps, fills)})
+    assert [r["class_id"] for r in rows] == expect
+    assert sum(r["tables"] for r in rows) == len(caps)
+    assert sum(r["keys"] for r in rows) == int(off[-1])
+    assert sum(r["queries"] for r in rows) == int(qoff[-1])
+    shapes = b.configs()
+    for r in rows:
+        assert r["fixed_shape"] == (16, 16)
+        # the fastest eligible configuration; the fixed one is beaten or was
+        # not eligible (more slots than the 10% rule allows)
+        assert r["chosen_ns"] <= r["fixed_ns"] or r["chosen_slots"] < r["fixed_slots"]
+    for t in range(len(caps)):
+        cls = (int(caps[t]).bit_length() - 1) // 2 * 4 + (0 if fills[t] < 0.6 else 1 if fills[t] < 0.7 else
+                                                          2 if fills[t] < 0.8 else 3)
+        assert shapes[b.table_config(t)] == rows[expect.index(cls)]["chosen_shape"]
+    assert np.all(from_dev(b.status_words(), np.uint32) == 0)   # left empty
+    for cfg in [launch, (64, 4, 64), (8192, 2, 512)]:
         b.set_launch(*cfg)
         b.init()
         st = dev_empty(K.numel(), 32)
         b.insert(K, V, O, status_out=st)
-        out, qst = dev_empty(Q.numel(), 32, 0xAB), dev_empty(Q.numel(), 32)
+        out, qst = dev_empty(Q.numel(), kb, 0xAB), dev_empty(Q.numel(), 32)
         b.retrieve(Q, QO, out, status_out=qst)
-        assert np.all(from_dev(st, np.uint32) == S_NONE)
-        gq = from_dev(qst, np.uint32)
-        got = from_dev(out, np.uint32)
-        for t in range(0, 300, 11):
-            q = queries[t]
-            sl = slice(int(qoff[t]), int(qoff[t + 1]))
-            lookup = dict(zip(keys[t].tolist(), vals[t].tolist()))
-            exp = np.array([lookup.get(x, 0xAB) for x in q.tolist()], np.uint32)
-            np.testing.assert_array_equal(got[sl], exp)
-            np.testing.assert_array_equal(gq[sl], np.where(np.isin(q, keys[t]), S_NONE, S_NOT_FOUND))
+        ist = from_dev(st, np.uint32)
+        per_table = [ist[int(off[t]):int(off[t + 1])] for t in range(len(caps))]
+        _check_against_oracle(oracle, b, kb, caps, keys, vals, queries, qoff, per_table, from_dev(out, dtype(kb)),
+                              from_dev(qst, np.uint32), from_dev(b.status_words(), np.uint32), every=13)
 
 
 def test_table_batch_full_configuration(cuda, oracle):

@@ -130,6 +130,53 @@ def test_capacity_rule():
     assert O.get_valid_capacity(17177758133 * 8 + 1, 8) == 0   # beyond the table
 
 
+def _is_prime(n):
+    if n < 2:
+        return False
+    d = 2
+    while d * d <= n:
+        if n % d == 0:
+            return False
+        d += 1
+    return True
+
+
+def test_batch_capacity_rule():
+    """TableBatch's small-table rule by trial division: the smallest prime
+    >= max(2, ceil(min / window)) windows of `window` slots."""
+    rng = np.random.default_rng(9)
+    cases = [(1, 4), (4, 4), (8, 4), (9, 4), (64, 16), (65, 16), (100, 32), (65536, 16)]
+    cases += [(int(m), int(w)) for m, w in zip(np.exp2(rng.uniform(0, 17, 200)).astype(np.int64),
+                                               rng.choice([4, 8, 16], 200))]
+    for mn, w in cases:
+        x = max(2, -(-mn // w))
+        while not _is_prime(x):
+            x += 1
+        assert O.batch_capacity(mn, w) == x * w, (mn, w)
+
+
+@pytest.mark.parametrize("kb,window,mn", [(32, 4, 64), (32, 8, 100), (64, 16, 300), (32, 16, 64)])
+def test_sized_svht_fills_every_slot(kb, window, mn):
+    """A table of exactly batch_capacity slots: double hashing over a prime
+    number of windows reaches every window, so exactly `capacity` distinct
+    keys are stored and every further key reports probing_length_exceeded;
+    stored keys read back their values."""
+    cap = O.batch_capacity(mn, window)
+    o = O.SVHT(0, kb, kb, cg=window, exact_capacity=cap)
+    assert o.capacity() == cap
+    keys = np.arange(1, cap + 31, dtype=np.uint32 if kb == 32 else np.uint64) * 7919
+    st = o.insert(keys, keys + 1)
+    assert np.sum(st == O.S_NONE) == cap and np.sum(st == O.S_PLE) == 30
+    assert np.all(st[:cap] == O.S_NONE)   # sequential: the first `capacity` keys fit
+    assert o.size() == cap and o.status() == O.S_PLE
+    out, rst = o.retrieve(keys)
+    assert np.all(rst[:cap] == O.S_NONE) and np.all(out[:cap] == keys[:cap] + 1)
+    assert np.all(rst[cap:] == O.S_PLE)   # full table: no EMPTY slot ends the walk
+    for key in (1, 42, 1337):
+        seq = O.probe_sequence(O.AL_DOUBLE, kb, key, O.DEFAULT_SEED, cap, window, cap, cap // window + 1)
+        assert len(np.unique(seq)) == cap
+
+
 def test_double_hashing_worked_example():
     c = CF["double_hashing_ref"]
     for case in c["cases"]:

@@ -19,6 +19,13 @@
 // EMPTY lane; the window budget ends a key with probing_length_exceeded.
 // Results and statuses are those of the other kernel families; keys of one
 // batch interleave as keys of different groups do in the reference.
+//
+// SPL (slots per lane) 2, for 8-byte packed slots: a group of CG/2 lanes
+// reads a window of CG slots, each lane its two adjacent slots with one
+// 16-byte load. Window bit b of the hit/EMPTY masks is slot 2*b of the
+// window for b < CG/2 and slot 2*(b - CG/2) + 1 above, so an insert tries
+// the window's even slots before its odd ones: the lane order inside a
+// window is slot placement only, which results and statuses do not see.
 #pragma once
 
 #include "batched_probing.hpp"
@@ -51,10 +58,30 @@ WC_D std::uint64_t group_field(std::uint64_t m, unsigned shift) noexcept {
     else return (m >> shift) & ((std::uint64_t(1) << CG) - 1);
 }
 
+// Slot in its window of window-mask bit b (groups of G lanes, SPL slots each).
+template<unsigned G, unsigned SPL>
+WC_D unsigned owner_slot_of_bit(unsigned b) noexcept {
+    if constexpr (SPL == 1) return b;
+    else return (b & (G - 1)) * 2 + b / G;
+}
+
+// The SPL slots of one lane (SPL 2: one 16-byte load of two 8-byte slots).
+template<unsigned SPL, class T>
+WC_D void owner_load_pair(const T* p, T (&out)[SPL]) noexcept {
+    if constexpr (SPL == 1) {
+        out[0] = load_slot(p);
+    } else {
+        static_assert(SPL == 2 && sizeof(T) == 8, "two slots per lane: 8-byte slots");
+        const uint4 v = *reinterpret_cast<const uint4*>(p);
+        __builtin_memcpy(&out[0], &v, 8);
+        __builtin_memcpy(&out[1], reinterpret_cast<const char*>(&v) + 8, 8);
+    }
+}
+
 // Inserts of keys_in[first .. end) by the calling wavefront (all 64 lanes
 // call this with the same range). `lds` holds 64 entries of this wave.
 // Single-value cores only (insert_refill_range covers multi-value cores).
-template<class Core, class StatusHandler, class W, bool HasValues>
+template<class Core, class StatusHandler, class W, bool HasValues, unsigned SPL = 1>
 WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in,
                              const typename Core::value_type* __restrict__ values_in, index_t first, index_t end,
                              const Core& core, index_t budget,
@@ -66,9 +93,12 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
     using Cursor = WindowCursor<typename Core::probing_scheme_type>;
     using Entry = OwnerEntry<K, W>;
     using Bits = typename std::conditional<(Core::cg_size() <= 32), std::uint32_t, std::uint64_t>::type;
-    constexpr unsigned CG = unsigned(Core::cg_size());
+    constexpr unsigned WIN = unsigned(Core::cg_size());   // slots per window
+    constexpr unsigned CG = WIN / SPL;                     // lanes per group
     static_assert(CG >= 2 && CG <= 64 && (CG & (CG - 1)) == 0, "groups of 2..64 lanes");
     static_assert(!is_multi_value_core<Core>::value, "single-value cores only");
+    static_assert(SPL == 1 || (SPL == 2 && sizeof(Probe) == 8 && !std::is_same<Probe, K>::value),
+                  "two slots per lane: packed 8-byte slots");
     constexpr bool keep_seen = !std::is_same<Probe, K>::value;
     constexpr unsigned kFree = 0, kLoad = 1, kClaim = 2;
     const unsigned lane = lane_id();
@@ -164,38 +194,43 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // only the loaded words stay in registers while the loads are in
         // flight; the owners' entries are read again from LDS to compare
-        Probe probe[CG];
+        Probe probe[CG][SPL];
 #pragma unroll
         for (unsigned j = 0; j < CG; ++j) {
             const W ew = lds[shift + j].w;
             const W at = ew != kNoWindow<W> ? ew : W(0);
-            probe[j] = Core::load_insert_probe(slots + index_t(at) * CG + rank);
+            if constexpr (SPL == 1) probe[j][0] = Core::load_insert_probe(slots + index_t(at) * WIN + rank);
+            else owner_load_pair<SPL>(slots + index_t(at) * WIN + rank * SPL, probe[j]);
         }
-        // claiming owners: CAS on the lowest untried EMPTY lane
-        const unsigned lead = unsigned(ffs64(std::uint64_t(emp)) - 1);
+        // claiming owners: CAS on the lowest untried EMPTY slot
+        const unsigned lead = owner_slot_of_bit<CG, SPL>(unsigned(ffs64(std::uint64_t(emp)) - 1));
         typename Core::cas_word old{};
-        if (phase == kClaim) old = core.cas_issue(slots + index_t(w) * CG + lead, seen, key, val);
+        if (phase == kClaim) old = core.cas_issue(slots + index_t(w) * WIN + lead, seen, key, val);
         __builtin_amdgcn_wave_barrier();
 
-        // each window's hits and EMPTY lanes, extracted by its owner
+        // each window's hits and EMPTY slots, extracted by its owner
         Bits hit = 0, empty = 0;
 #pragma unroll
         for (unsigned j = 0; j < CG; ++j) {
             const Entry e = lds[shift + j];
             const bool on = e.w != kNoWindow<W>;
-            const K tk = Core::probe_key(probe[j]);
-            const std::uint64_t hm = ballot(on && tk == e.key);
-            const std::uint64_t em = ballot(on && Core::is_empty_key(tk));
-            if (rank == j) {
-                hit = Bits(group_field<CG>(hm, shift));
-                empty = Bits(group_field<CG>(em, shift));
+#pragma unroll
+            for (unsigned h = 0; h < SPL; ++h) {
+                const K tk = Core::probe_key(probe[j][h]);
+                const std::uint64_t hm = ballot(on && tk == e.key);
+                const std::uint64_t em = ballot(on && Core::is_empty_key(tk));
+                if (rank == j) {
+                    hit |= Bits(group_field<CG>(hm, shift) << (h * CG));
+                    empty |= Bits(group_field<CG>(em, shift) << (h * CG));
+                }
             }
         }
 
         if (phase == kLoad) {
             if (hit != 0) {   // duplicate: the value is overwritten in place
                 Status st = Status::duplicate_key();
-                core.finish_insert(index_t(w) * CG + unsigned(ffs64(std::uint64_t(hit)) - 1), val, false, st, join);
+                core.finish_insert(index_t(w) * WIN + owner_slot_of_bit<CG, SPL>(unsigned(ffs64(std::uint64_t(hit)) - 1)),
+                                   val, false, st, join);
                 join(Status::duplicate_key());
                 StatusHandler::handle(st, status_out, pos);
                 phase = kFree;
@@ -210,7 +245,7 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
             const int code = core.cas_outcome(old, seen, key);
             if (code == 1 || code == 2) {   // claimed, or the key was already there
                 Status st = code == 1 ? Status::none() : Status::duplicate_key();
-                core.finish_insert(index_t(w) * CG + lead, val, code == 1 && Core::publishes_value, st, join);
+                core.finish_insert(index_t(w) * WIN + lead, val, code == 1 && Core::publishes_value, st, join);
                 if (code == 2) join(Status::duplicate_key());
                 StatusHandler::handle(st, status_out, pos);
                 phase = kFree;
@@ -234,7 +269,7 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
 // cycle of windows and the wavefront has seen a window with an EMPTY slot
 // (the table is static during the call), where the walk would end at an
 // EMPTY slot with key_not_found.
-template<class Core, class StatusHandler, class W>
+template<class Core, class StatusHandler, class W, unsigned SPL = 1>
 WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_in, index_t first, index_t end,
                                typename Core::out_type* __restrict__ values_out, const Core& core, index_t budget,
                                typename StatusHandler::base_type* __restrict__ status_out,
@@ -244,9 +279,11 @@ WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_
     using Slot = typename Core::probe_slot_type;
     using Cursor = WindowCursor<typename Core::probing_scheme_type>;
     using Entry = OwnerEntry<K, W>;
-    constexpr unsigned CG = unsigned(Core::cg_size());
+    constexpr unsigned WIN = unsigned(Core::cg_size());   // slots per window
+    constexpr unsigned CG = WIN / SPL;                     // lanes per group
     static_assert(CG >= 2 && CG <= 64 && (CG & (CG - 1)) == 0, "groups of 2..64 lanes");
     static_assert(!is_multi_value_core<Core>::value, "single-value cores only");
+    static_assert(SPL == 1 || (SPL == 2 && sizeof(Slot) == 8), "two slots per lane: 8-byte slots");
     constexpr unsigned kFree = 0, kLoad = 1, kFilter = 2;
     const unsigned lane = lane_id();
     const unsigned rank = lane & (CG - 1);
@@ -255,7 +292,7 @@ WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_
 
     const probing_schemes::Geometry& geo = core.geometry();
     const W p = W(geo.windows);
-    const Slot* __restrict__ lane_slots = core.slot_ptr() + rank;
+    const Slot* __restrict__ lane_slots = core.slot_ptr() + rank * SPL;
     const std::uint32_t* filter = nullptr;
     if constexpr (has_overflow_filter<Core>::value)
         if (budget >= index_t(geo.windows)) filter = core.filter;
@@ -319,12 +356,12 @@ WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // only the loaded slots stay in registers while the loads are in
         // flight; the owners' entries are read again from LDS to compare
-        Slot s[CG];
+        Slot s[CG][SPL];
 #pragma unroll
         for (unsigned j = 0; j < CG; ++j) {
             const W ew = lds[shift + j].w;
             const W at = ew != kNoWindow<W> ? ew : W(0);
-            s[j] = load_slot(lane_slots + index_t(at) * CG);
+            owner_load_pair<SPL>(lane_slots + index_t(at) * WIN, s[j]);
         }
         std::uint32_t fword = 0;
         if (phase == kFilter) fword = filter[w >> 2];
@@ -335,15 +372,18 @@ WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_
         for (unsigned j = 0; j < CG; ++j) {
             const Entry e = lds[shift + j];
             const bool on = e.w != kNoWindow<W>;
-            const K tk = Core::key_of(s[j]);
-            const bool h = on && tk == e.key;
-            if (h) values_out[first + e.pos] = core.found(Core::value_of(s[j]));   // the slot's lane writes the hit
-            const std::uint64_t hm = ballot(h);
-            const std::uint64_t em = ballot(on && Core::is_empty_key(tk));
-            any_empty |= em;
-            if (rank == j) {
-                hit = group_field<CG>(hm, shift) != 0;
-                empty = group_field<CG>(em, shift) != 0;
+#pragma unroll
+            for (unsigned x = 0; x < SPL; ++x) {
+                const K tk = Core::key_of(s[j][x]);
+                const bool h = on && tk == e.key;
+                if (h) values_out[first + e.pos] = core.found(Core::value_of(s[j][x]));   // the slot's lane writes the hit
+                const std::uint64_t hm = ballot(h);
+                const std::uint64_t em = ballot(on && Core::is_empty_key(tk));
+                any_empty |= em;
+                if (rank == j) {
+                    hit = hit || group_field<CG>(hm, shift) != 0;
+                    empty = empty || group_field<CG>(em, shift) != 0;
+                }
             }
         }
         seen_empty = seen_empty || any_empty != 0;

@@ -708,6 +708,17 @@ WC_H unsigned refill_grid() noexcept {
     return grid;
 }
 
+// Grid of a lane-owner launch over num_in keys: the resident grid, or
+// fewer blocks when that would leave lanes without a key (each wavefront
+// takes an equal share, so a share under 64 keys idles lanes for the whole
+// launch).
+template<auto Kernel>
+WC_H unsigned owner_grid(index_t num_in) noexcept {
+    const index_t full = refill_grid<Kernel>();
+    const index_t needed = SDIV(num_in, index_t(block_size()));   // one key per lane
+    return unsigned(needed < 1 ? 1 : needed < full ? needed : full);
+}
+
 template<class Core, class StatusHandler, int U, bool HasValues>
 WC_H void launch_insert_refill_as(const typename Core::key_type* keys_in, const typename Core::value_type* values_in,
                                   index_t num_in, const Core& core, index_t probing_length,
@@ -731,12 +742,12 @@ WC_H void launch_insert_owner(const typename Core::key_type* keys_in, const type
                               const index_t* num_dev) noexcept {
     if (core.geometry().windows < (index_t(1) << 31)) {
         constexpr auto k = kernels::insert_owner<Core, StatusHandler, std::uint32_t, HasValues>;
-        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out,
-                                                         num_dev);
+        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length,
+                                                              status_out, num_dev);
     } else {
         constexpr auto k = kernels::insert_owner<Core, StatusHandler, index_t, HasValues>;
-        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length, status_out,
-                                                         num_dev);
+        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length,
+                                                              status_out, num_dev);
     }
 }
 
@@ -746,10 +757,12 @@ WC_H void launch_retrieve_owner(const typename Core::key_type* keys_in, index_t 
                                 typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
     if (core.geometry().windows < (index_t(1) << 31)) {
         constexpr auto k = kernels::retrieve_owner<Core, StatusHandler, std::uint32_t>;
-        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
+        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length,
+                                                              status_out);
     } else {
         constexpr auto k = kernels::retrieve_owner<Core, StatusHandler, index_t>;
-        k<<<refill_grid<k>(), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length, status_out);
+        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length,
+                                                              status_out);
     }
 }
 

@@ -1,39 +1,62 @@
 // TableBatch: many small single-value hash tables in one slot arena, all
-// inserted into or looked up in one persistent launch.
+// inserted into or looked up in one persistent launch, with the probe
+// geometry of every table chosen by an autotuner.
 //
 // No reference counterpart: warpcore builds one table per object, and its
 // capacity rule (first prime >= 131072 windows) makes every table at least
-// 131101 * cg slots. A batch of thousands of small tables therefore has its
-// own rule: table t gets p_t = the smallest prime >= max(2, SDIV(min_t, cg))
-// windows of cg slots, so a 64-slot request is a few windows, not a
-// megabyte. Each table is otherwise a SingleValueHashTable<Key, Value> with
-// double hashing over aligned windows: the same hashes, insert protocol,
-// statuses and per-table status word, so per-key results equal those of a
-// plain table fed the same keys (only slot placement differs).
+// 131101 * cg slots. A batch of thousands of small tables has its own rule:
+// table t, probed in windows of W_t slots, gets p_t = the smallest prime
+// >= max(2, SDIV(min_t, W_t)) windows, so a 64-slot request is a few
+// windows, not a megabyte. Each table is otherwise a SingleValueHashTable
+// <Key, Value> with double hashing over aligned windows of W_t slots: the
+// same hashes, insert protocol, statuses and per-table status word, so
+// per-key results equal those of a table of that geometry fed the same keys
+// (only slot placement differs).
+//
+// Configurations. A table's configuration is (lanes, window): groups of
+// `lanes` lanes read windows of `window` slots, one slot per lane or, for
+// 8-byte slots, two adjacent slots per lane with one 16-byte load
+// (owner_probing.hpp). 8-byte slots: (4,4) (4,8) (8,8) (8,16) (16,16);
+// 16-byte slots: (4,4) (8,8) (16,16).
 //
 // A bulk call takes the keys of all tables concatenated, table t owning
 // keys [offsets[t], offsets[t+1]). The call cuts every table's range into
-// work items of `item_keys` keys, lists them on the device (count, scan,
-// fill), and one persistent kernel lets each cooperative group take items
-// from an atomic counter (work stealing): small and large tables mix
-// freely and no group idles while items remain. Inside an item the group
-// runs the refill kernels' per-range code against that table's view.
-// Launch parameters (keys in flight per group, item size, blocks per CU)
-// come from a BatchLaunch; the C ABI's tuner times candidates on the
-// caller's workload and keeps the fastest.
+// work items of up to `item_keys` keys and lists them on the device (count,
+// scan, fill) in the batch's table order (capacity descending, so the
+// smallest items come last); one persistent kernel lets each wavefront take
+// items from an atomic counter (work stealing) and run the lane-owner code
+// of the item's configuration on its table, 64 keys in flight per
+// wavefront. Launch parameters (keys per item, blocks per CU, block size)
+// come from a BatchLaunch. When every table has the same configuration the
+// launch uses a kernel compiled for that configuration alone (registers of
+// that configuration only).
+//
+// Autotuner (tune()). Tables fall into classes by requested capacity
+// (bands [4^k, 4^(k+1)) slots) and fill (keys over requested capacity:
+// below 0.6, 0.6-0.7, 0.7-0.8, from 0.8); the key width is the batch's.
+// For every configuration the tuner lays all tables out with it and times
+// each class alone: insert, then retrieve, of the caller's workload
+// restricted to the class's tables. Each class keeps its fastest
+// configuration among those whose slot total for the class is within 10%
+// of the smallest (a wide window rounds a tiny table up to more slots).
+// With the chosen configurations laid out, the tuner times the launch
+// candidates on the whole workload and keeps the fastest. Tuning leaves the
+// batch empty.
 #pragma once
 
 #include "base.hpp"
 #include "capacity.hpp"
 #include "defaults.hpp"
 #include "engine.hpp"
-#include "lockstep_probing.hpp"
-#include "refill_probing.hpp"
+#include "owner_probing.hpp"
 #include "scan.hpp"
 #include "single_value_hash_table.hpp"
 #include "status.hpp"
 #include "storage.hpp"
 
+#include <algorithm>
+#include <numeric>
+#include <type_traits>
 #include <vector>
 
 namespace warpcore {
@@ -42,159 +65,278 @@ namespace detail {
 
 // Per-table geometry in device memory.
 struct BatchTableDesc {
-    index_t slot_offset;
+    index_t slot_offset;   // a multiple of the table's window
     probing_schemes::Geometry geo;
+    std::uint32_t config;
+};
+
+// (lanes, window) choices by slot size
+template<std::size_t SlotBytes>
+struct BatchConfigs;
+template<>
+struct BatchConfigs<8> {
+    static constexpr int count = 5;
+    static constexpr unsigned lanes[count] = {4, 4, 8, 8, 16};
+    static constexpr unsigned window[count] = {4, 8, 8, 16, 16};
+};
+template<>
+struct BatchConfigs<16> {
+    static constexpr int count = 3;
+    static constexpr unsigned lanes[count] = {4, 8, 16};
+    static constexpr unsigned window[count] = {4, 8, 16};
 };
 
 // small-table capacity rule (see the file comment)
-WC_H index_t batch_table_windows(index_t min_capacity, index_t cg) noexcept {
-    const index_t w = SDIV(min_capacity, cg);
+WC_H index_t batch_table_windows(index_t min_capacity, index_t window) noexcept {
+    const index_t w = SDIV(min_capacity, window);
     return next_prime_at_least(w < 2 ? 2 : w);
 }
 
 struct BatchLaunch {
-    int keys_per_batch;   // keys in flight per group (U of the refill code): 4 or 8
-    index_t item_keys;    // keys per work item
-    int blocks_per_cu;    // persistent grid: CUs * blocks_per_cu blocks of 256
+    index_t item_keys;   // keys per work item
+    int blocks_per_cu;   // persistent grid: CUs * blocks_per_cu blocks
+    int block_size;      // threads per block: 64..512, a multiple of 64
 };
+
+// Tuning class of a table: capacity band * 4 + fill band.
+WC_H std::uint32_t batch_tune_class(index_t min_capacity, index_t keys) noexcept {
+    const unsigned lg = min_capacity ? unsigned(63 - __builtin_clzll(min_capacity)) : 0u;
+    const unsigned band = lg / 2 < 15 ? lg / 2 : 15u;
+    const double fill = min_capacity ? double(keys) / double(min_capacity) : 1.0;
+    const unsigned f = fill < 0.6 ? 0u : fill < 0.7 ? 1u : fill < 0.8 ? 2u : 3u;
+    return band * 4 + f;
+}
+
+// One class of a tuning run.
+struct BatchTuneClass {
+    std::uint32_t id;   // capacity band * 4 + fill band
+    index_t tables, keys, queries;
+    int fixed_config, chosen_config;
+    double fixed_ms, chosen_ms;   // the class alone: insert + retrieve
+    index_t fixed_slots, chosen_slots;
+};
+
+struct BatchTuneReport {
+    std::vector<BatchTuneClass> classes;
+    double fixed_ms = 0.0;   // whole workload: every table at the fixed configuration, default launch
+    double tuned_ms = 0.0;   // whole workload: chosen configurations and launch
+    BatchLaunch launch{};
+};
+
+WC_D index_t wave_first(index_t v) noexcept {
+    const unsigned lo = __builtin_amdgcn_readfirstlane(unsigned(v));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(unsigned(v >> 32));
+    return (index_t(hi) << 32) | lo;
+}
+
+template<int I, int N, class F>
+WC_D void batch_dispatch(unsigned config, F& f) {
+    if constexpr (I < N) {
+        if (config == unsigned(I)) f(std::integral_constant<int, I>{});
+        else batch_dispatch<I + 1, N>(config, f);
+    }
+}
 
 }  // namespace detail
 
 namespace kernels {
 
-// items[t] = SDIV(n_t, item_keys)
+// items[i] = SDIV(ends[t] - begins[t], item_keys) for t = order[i]
 template<class T>
-__global__ __launch_bounds__(256) void batch_count_items(const T* __restrict__ offsets, index_t num_tables,
-                                                         index_t item_keys, T* __restrict__ items) {
+__global__ __launch_bounds__(256) void batch_count_items(const T* __restrict__ begins, const T* __restrict__ ends,
+                                                         const std::uint32_t* __restrict__ order, index_t n,
+                                                         T item_keys, T* __restrict__ items) {
     const index_t stride = index_t(gridDim.x) * blockDim.x;
-    for (index_t t = index_t(blockIdx.x) * blockDim.x + threadIdx.x; t < num_tables; t += stride)
-        items[t] = SDIV(offsets[t + 1] - offsets[t], item_keys);
-}
-
-// item_table[first_t + j] = t for j < items[t]; first = exclusive scan of items
-template<class T>
-__global__ __launch_bounds__(256) void batch_fill_items(const T* __restrict__ first, const T* __restrict__ offsets,
-                                                        index_t num_tables, index_t item_keys,
-                                                        std::uint32_t* __restrict__ item_table) {
-    const index_t stride = index_t(gridDim.x) * blockDim.x;
-    for (index_t t = index_t(blockIdx.x) * blockDim.x + threadIdx.x; t < num_tables; t += stride) {
-        const index_t n = SDIV(offsets[t + 1] - offsets[t], item_keys);
-        for (index_t j = 0; j < n; ++j) item_table[first[t] + j] = std::uint32_t(t);
+    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const std::uint32_t t = order[i];
+        items[i] = SDIV(ends[t] - begins[t], item_keys);
     }
 }
 
-template<class View, class StatusHandler, int U, bool IsInsert>
-__global__ __launch_bounds__(256) void batch_persistent(
-    const typename View::key_type* __restrict__ keys_in, const typename View::value_type* __restrict__ values_in,
-    typename View::value_type* __restrict__ values_out, const index_t* __restrict__ offsets, index_t num_tables,
-    const index_t* __restrict__ first_item, const std::uint32_t* __restrict__ item_table, index_t item_keys,
-    index_t* __restrict__ counter, const detail::BatchTableDesc* __restrict__ tables,
-    typename View::slot_type* slots, Status* __restrict__ statuses, typename View::key_type seed,
-    index_t probing_length, typename StatusHandler::base_type* __restrict__ status_out) {
-    constexpr unsigned CG = unsigned(View::cg_size());
-    using W = std::uint32_t;
-    using Stage = typename std::conditional<IsInsert,
-                                            detail::RefillInsertEntry<typename View::key_type, W,
-                                                                      typename View::value_type>,
-                                            detail::RefillEntry<typename View::key_type, W>>::type;
-    __shared__ Stage stage[256];
-    auto* my = stage + (threadIdx.x & ~(CG - 1));
-    const unsigned rank = lane_id() & (CG - 1);
-    const index_t total = num_tables ? first_item[num_tables - 1] +
-                                           SDIV(offsets[num_tables] - offsets[num_tables - 1], item_keys)
-                                     : 0;
+// The items of position i (table t = order[i]) from first[i] on: table t
+// and the first key of each. first = exclusive scan of items.
+template<class T>
+__global__ __launch_bounds__(256) void batch_fill_items(const T* __restrict__ first, const T* __restrict__ begins,
+                                                        const T* __restrict__ ends,
+                                                        const std::uint32_t* __restrict__ order, index_t n,
+                                                        T item_keys, std::uint32_t* __restrict__ item_table,
+                                                        T* __restrict__ item_begin) {
+    const index_t stride = index_t(gridDim.x) * blockDim.x;
+    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const std::uint32_t t = order[i];
+        const T b = begins[t];
+        const T m = SDIV(ends[t] - b, item_keys);
+        for (T j = 0; j < m; ++j) {
+            item_table[first[i] + j] = t;
+            item_begin[first[i] + j] = b + j * item_keys;
+        }
+    }
+}
+
+// Persistent lane-owner kernel over the work items: each wavefront takes
+// items until none is left. Only >= 0: every table has that configuration.
+template<class Key, class Value, Key EmptyKey, Key TombstoneKey, class StatusHandler, bool IsInsert, int Only>
+__global__ __launch_bounds__(512) void batch_owner(
+    const Key* __restrict__ keys_in, const Value* __restrict__ values_in, Value* __restrict__ values_out,
+    const index_t* __restrict__ ends, const index_t* __restrict__ items, const index_t* __restrict__ first_item,
+    index_t num_tables, const std::uint32_t* __restrict__ item_table, const index_t* __restrict__ item_begin,
+    index_t item_keys, index_t* __restrict__ counter, const detail::BatchTableDesc* __restrict__ tables,
+    storage::key_value::pair_t<Key, Value>* slots, Status* __restrict__ statuses, Key seed, index_t probing_length,
+    typename StatusHandler::base_type* __restrict__ status_out) {
+    using Slot = storage::key_value::pair_t<Key, Value>;
+    using Configs = detail::BatchConfigs<sizeof(Slot)>;
+    __shared__ detail::OwnerEntry<Key, std::uint32_t> lds[512];
+    auto* my = lds + (threadIdx.x & ~63u);
+    const index_t total = num_tables ? first_item[num_tables - 1] + items[num_tables - 1] : 0;
     while (true) {
         index_t item = 0;
-        if (rank == 0) item = detail::atomic_add(counter, index_t(1));
-        item = tile<CG>::shfl(item, 0);
+        if (lane_id() == 0) item = detail::atomic_add(counter, index_t(1));
+        item = detail::wave_first(item);
         if (item >= total) break;
-        const index_t t = item_table[item];
-        const index_t j = item - first_item[t];
-        const index_t begin = offsets[t] + j * item_keys;
-        const index_t end_t = offsets[t + 1];
-        const index_t end = begin + item_keys < end_t ? begin + item_keys : end_t;
+        const std::uint32_t t = item_table[item];
+        const index_t begin = item_begin[item];
+        const index_t tend = ends[t];
+        const index_t end = tend - begin > item_keys ? begin + item_keys : tend;
         const detail::BatchTableDesc d = tables[t];
-        const View view{slots + d.slot_offset, statuses + t, d.geo, seed, true};
-        const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, d.geo.windows);
-        detail::DeferredJoin join;
-        if constexpr (IsInsert)
-            detail::insert_refill_range<View, StatusHandler, U, W, true>(keys_in, values_in, begin, end, view, budget,
-                                                                         status_out, my, join);
-        else
-            detail::retrieve_refill_range<View, StatusHandler, U, W>(keys_in, begin, end, values_out, view, budget,
-                                                                     status_out, my, join);
-        const status_base_t all = tile<CG>::reduce_or(join.acc);
-        if (rank == 0 && all != 0) statuses[t].atomic_join(Status::from_base(all));
+        auto run = [&](auto ic) {
+            constexpr int I = decltype(ic)::value;
+            constexpr unsigned WIN = Configs::window[I];
+            constexpr unsigned SPL = WIN / Configs::lanes[I];
+            using Scheme = defaults::probing_scheme_t<Key, WIN>;
+            using View = detail::SingleValueView<Key, Value, EmptyKey, TombstoneKey, Scheme, Slot>;
+            const View view{slots + d.slot_offset, statuses + t, d.geo, seed, true, nullptr};
+            const index_t budget = probing_schemes::detail::window_budget(probing_length, WIN, d.geo.windows);
+            detail::DeferredJoin join;
+            if constexpr (IsInsert)
+                detail::insert_owner_range<View, StatusHandler, std::uint32_t, true, SPL>(
+                    keys_in, values_in, begin, end, view, budget, status_out, my, join);
+            else
+                detail::retrieve_owner_range<View, StatusHandler, std::uint32_t, SPL>(
+                    keys_in, begin, end, values_out, view, budget, status_out, my, join);
+            join.flush(statuses + t);
+        };
+        if constexpr (Only >= 0) run(std::integral_constant<int, Only>{});
+        else detail::batch_dispatch<0, Configs::count>(d.config, run);
     }
 }
 
 }  // namespace kernels
 
-template<class Key, class Value, index_t CGSize = 16, Key EmptyKey = defaults::empty_key<Key>(),
+template<class Key, class Value, Key EmptyKey = defaults::empty_key<Key>(),
          Key TombstoneKey = defaults::tombstone_key<Key>()>
 class TableBatch {
     static_assert(checks::is_valid_key_type<Key>(), "invalid key type");
-    static_assert(checks::is_valid_cg_size(CGSize), "invalid cooperative group size");
 
 public:
     using key_type = Key;
     using value_type = Value;
     using index_type = index_t;
     using status_type = Status;
-    using probing_scheme_type = defaults::probing_scheme_t<Key, CGSize>;
     using slot_type = storage::key_value::pair_t<Key, Value>;
-    using view_type = detail::SingleValueView<Key, Value, EmptyKey, TombstoneKey, probing_scheme_type, slot_type>;
+    using configs = detail::BatchConfigs<sizeof(slot_type)>;
     using launch_type = detail::BatchLaunch;
+    using tune_report_type = detail::BatchTuneReport;
 
-    WC_HD static constexpr index_type cg_size() noexcept { return CGSize; }
+    WC_H static constexpr int num_configs() noexcept { return configs::count; }
+    WC_H static unsigned config_lanes(int c) noexcept { return configs::lanes[c]; }
+    WC_H static unsigned config_window(int c) noexcept { return configs::window[c]; }
+    // index of configuration (lanes, window), or -1
+    WC_H static int config_index(unsigned lanes, unsigned window) noexcept {
+        for (int c = 0; c < configs::count; ++c)
+            if (configs::lanes[c] == lanes && configs::window[c] == window) return c;
+        return -1;
+    }
+    WC_H static launch_type default_launch() noexcept { return launch_type{2048, 8, 256}; }
 
-    // min_capacities: host array of num_tables requested capacities (slots)
+    // min_capacities: host array of num_tables requested capacities (slots);
+    // every table starts with configuration `config`
     WC_H TableBatch(const index_type* min_capacities, index_type num_tables,
-                    key_type seed = defaults::seed<key_type>()) noexcept
-        : num_tables_(num_tables), seed_(seed) {
-        std::vector<detail::BatchTableDesc> h(num_tables);
-        index_type slots = 0;
-        bool small = true;   // the persistent kernel keeps window indices in 32 bits
-        for (index_type t = 0; t < num_tables; ++t) {
-            const index_type p = detail::batch_table_windows(min_capacities[t], CGSize);
-            small = small && p < (index_type(1) << 31);
-            h[t].slot_offset = slots;
-            h[t].geo = probing_schemes::Geometry::make(p * CGSize, CGSize);
-            slots += p * CGSize;
-        }
-        total_slots_ = slots;
-        capacities_.resize(num_tables);
-        for (index_type t = 0; t < num_tables; ++t) capacities_[t] = h[t].geo.capacity;
-        ok_ = small && num_tables > 0 && malloc_(&tables_, sizeof(detail::BatchTableDesc) * num_tables) &&
-              malloc_(&slots_, sizeof(slot_type) * total_slots_) && malloc_(&statuses_, sizeof(Status) * num_tables) &&
+                    key_type seed = defaults::seed<key_type>(), int config = config_index(16, 16)) noexcept
+        : num_tables_(num_tables), seed_(seed), min_caps_(min_capacities, min_capacities + num_tables),
+          fixed_config_(config) {
+        ok_ = num_tables > 0 && num_tables < (index_type(1) << 32) && config >= 0 && config < configs::count &&
+              malloc_(&tables_, sizeof(detail::BatchTableDesc) * num_tables) &&
+              malloc_(&statuses_, sizeof(Status) * num_tables) &&
+              malloc_(&order_, sizeof(std::uint32_t) * num_tables) &&
               malloc_(&parts_, sizeof(index_type) * detail::kScanParts) && malloc_(&counter_, sizeof(index_type)) &&
-              malloc_(&items_, sizeof(index_type) * num_tables) && malloc_(&first_item_, sizeof(index_type) * num_tables);
-        if (ok_)
-            ok_ = hipMemcpy(tables_, h.data(), sizeof(detail::BatchTableDesc) * num_tables, hipMemcpyHostToDevice) ==
-                  hipSuccess;
-        if (ok_) init();
+              malloc_(&items_, sizeof(index_type) * num_tables) &&
+              malloc_(&first_item_, sizeof(index_type) * num_tables);
+        if (ok_) ok_ = set_config(config);
     }
     TableBatch(const TableBatch&) = delete;
     TableBatch& operator=(const TableBatch&) = delete;
     WC_H ~TableBatch() noexcept {
-        for (void* p : {(void*)tables_, (void*)slots_, (void*)statuses_, (void*)parts_, (void*)counter_,
-                        (void*)items_, (void*)first_item_, (void*)item_table_})
+        for (void* p : {(void*)tables_, (void*)slots_, (void*)statuses_, (void*)order_, (void*)parts_,
+                        (void*)counter_, (void*)items_, (void*)first_item_, (void*)item_table_, (void*)item_begin_})
             if (p) (void)hipFree(p);
     }
 
     WC_H bool ok() const noexcept { return ok_; }
     WC_H index_type num_tables() const noexcept { return num_tables_; }
     WC_H index_type capacity(index_type t) const noexcept { return capacities_[t]; }
+    WC_H int table_config(index_type t) const noexcept { return int(config_of_[t]); }
+    WC_H int fixed_config() const noexcept { return fixed_config_; }
     WC_H index_type total_slots() const noexcept { return total_slots_; }
     WC_H index_type bytes_total() const noexcept {
         return total_slots_ * sizeof(slot_type) + num_tables_ * (sizeof(detail::BatchTableDesc) + sizeof(Status));
     }
 
+    // Lays every table out with configuration c and empties the batch.
+    WC_H bool set_config(int c) noexcept {
+        if (c < 0 || c >= configs::count) return false;
+        std::vector<std::uint8_t> all(num_tables_, std::uint8_t(c));
+        return set_configs(all.data());
+    }
+    // Lays table t out with configuration per_table[t] (host array) and
+    // empties the batch. False (batch unchanged) on an unknown configuration
+    // or when the arena cannot grow.
+    WC_H bool set_configs(const std::uint8_t* per_table) noexcept {
+        for (index_type t = 0; t < num_tables_; ++t)
+            if (per_table[t] >= configs::count) return false;
+        std::vector<detail::BatchTableDesc> h(num_tables_);
+        std::vector<index_type> caps(num_tables_);
+        index_type slots = 0;
+        for (index_type t = 0; t < num_tables_; ++t) {
+            const index_type win = configs::window[per_table[t]];
+            const index_type p = detail::batch_table_windows(min_caps_[t], win);
+            if (p >= (index_type(1) << 31)) return false;   // window indices are 32-bit in the kernel
+            slots = SDIV(slots, win) * win;
+            h[t].slot_offset = slots;
+            h[t].geo = probing_schemes::Geometry::make(p * win, win);
+            h[t].config = per_table[t];
+            caps[t] = p * win;
+            slots += p * win;
+        }
+        if (slots > arena_slots_) {
+            slot_type* grown = nullptr;
+            if (!malloc_(&grown, sizeof(slot_type) * slots)) return false;
+            if (slots_) (void)hipFree(slots_);
+            slots_ = grown;
+            arena_slots_ = slots;
+        }
+        std::vector<std::uint32_t> order(num_tables_);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](std::uint32_t a, std::uint32_t b) { return caps[a] > caps[b]; });
+        if (hipMemcpy(tables_, h.data(), sizeof(detail::BatchTableDesc) * num_tables_, hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(order_, order.data(), sizeof(std::uint32_t) * num_tables_, hipMemcpyHostToDevice) != hipSuccess)
+            return false;
+        capacities_ = std::move(caps);
+        config_of_.assign(per_table, per_table + num_tables_);
+        total_slots_ = slots;
+        single_config_ = int(per_table[0]);
+        for (index_type t = 1; t < num_tables_; ++t)
+            if (per_table[t] != per_table[0]) single_config_ = -1;
+        init();
+        return hipStreamSynchronize(nullptr) == hipSuccess;
+    }
+
     // every slot EMPTY, every status word none
     WC_H void init(hipStream_t stream = 0) noexcept {
-        if (!ok_) return;
-        storage::kernels::fill_pairs<<<detail::grid_for(total_slots_, 1, 8), block_size(), 0, stream>>>(slots_, total_slots_,
-                                                                                              EmptyKey, Value{});
+        if (!slots_) return;
+        storage::kernels::fill_pairs<<<detail::grid_for(total_slots_, 1, 8), block_size(), 0, stream>>>(
+            slots_, total_slots_, EmptyKey, Value{});
         storage::kernels::fill<std::uint32_t><<<detail::grid_for(num_tables_, 1, 8), block_size(), 0, stream>>>(
             reinterpret_cast<std::uint32_t*>(statuses_), num_tables_, 0u);
     }
@@ -207,26 +349,41 @@ public:
                      index_type probing_length = defaults::probing_length(),
                      typename StatusHandler::base_type* status_out = nullptr,
                      const launch_type* launch = nullptr) noexcept {
-        run<StatusHandler, true>(keys_in, values_in, nullptr, offsets, num_in, stream, probing_length, status_out,
-                                 launch ? *launch : default_launch());
+        run<StatusHandler, true>(keys_in, values_in, nullptr, offsets, offsets + 1, num_in, stream, probing_length,
+                                 status_out, launch ? *launch : launch_);
     }
     template<class StatusHandler = defaults::status_handler_t>
     WC_H void retrieve(const key_type* keys_in, const index_type* offsets, index_type num_in,
                        value_type* values_out, hipStream_t stream = 0,
                        index_type probing_length = defaults::probing_length(),
                        typename StatusHandler::base_type* status_out = nullptr,
-                       const launch_type* launch = nullptr) const noexcept {
-        const_cast<TableBatch*>(this)->template run<StatusHandler, false>(
-            keys_in, nullptr, values_out, offsets, num_in, stream, probing_length, status_out,
-            launch ? *launch : default_launch());
+                       const launch_type* launch = nullptr) noexcept {
+        run<StatusHandler, false>(keys_in, nullptr, values_out, offsets, offsets + 1, num_in, stream, probing_length,
+                                  status_out, launch ? *launch : launch_);
+    }
+
+    // Autotuner (see the file comment). keys/values/offsets: an insert
+    // workload, queries/query_offsets (num_queries may be 0): a retrieve
+    // workload, values_out: num_queries outputs. Returns 0 or a HIP error;
+    // the batch is left empty with the chosen configurations and launch.
+    WC_H int tune(const key_type* keys_in, const value_type* values_in, const index_type* offsets,
+                  index_type num_in, const key_type* queries, const index_type* query_offsets,
+                  index_type num_queries, value_type* values_out, hipStream_t stream,
+                  tune_report_type* report = nullptr) noexcept;
+
+    WC_H const launch_type& launch() const noexcept { return launch_; }
+    WC_H bool set_launch(const launch_type& l) noexcept {
+        if (l.item_keys < 1 || l.blocks_per_cu < 1 || l.block_size < 64 || l.block_size > 512 ||
+            l.block_size % 64 != 0)
+            return false;
+        launch_ = l;
+        return true;
     }
 
     // per-table status words (device array of num_tables)
     WC_H const Status* status_words() const noexcept { return statuses_; }
     WC_H slot_type* slots() const noexcept { return slots_; }
     WC_H int last_error() const noexcept { return last_error_; }
-
-    WC_H static launch_type default_launch() noexcept { return launch_type{8, 64, 8}; }
 
 private:
     template<class T>
@@ -237,63 +394,265 @@ private:
 
     template<class StatusHandler, bool IsInsert>
     WC_H void run(const key_type* keys_in, const value_type* values_in, value_type* values_out,
-                  const index_type* offsets, index_type num_in, hipStream_t stream, index_type probing_length,
-                  typename StatusHandler::base_type* status_out, launch_type launch) noexcept {
+                  const index_type* begins, const index_type* ends, index_type num_in, hipStream_t stream,
+                  index_type probing_length, typename StatusHandler::base_type* status_out,
+                  launch_type launch) noexcept {
         last_error_ = 0;
         if (!ok_ || num_in == 0) return;
-        const index_type ik = launch.item_keys < 1 ? 1 : launch.item_keys;
+        if (launch.item_keys < 1 || launch.blocks_per_cu < 1 || launch.block_size < 64 || launch.block_size > 512 ||
+            launch.block_size % 64 != 0) {
+            last_error_ = int(hipErrorInvalidValue);
+            return;
+        }
+        const index_type ik = launch.item_keys;
         // items: at most SDIV(num_in, ik) + num_tables
         const index_type max_items = SDIV(num_in, ik) + num_tables_;
         if (max_items > item_capacity_) {
-            if (item_table_) (void)hipFree(item_table_);
+            for (void* p : {(void*)item_table_, (void*)item_begin_})
+                if (p) (void)hipFree(p);
             item_table_ = nullptr;
+            item_begin_ = nullptr;
             item_capacity_ = 0;
-            if (!malloc_(&item_table_, sizeof(std::uint32_t) * max_items)) {
+            if (!malloc_(&item_table_, sizeof(std::uint32_t) * max_items) ||
+                !malloc_(&item_begin_, sizeof(index_type) * max_items)) {
                 last_error_ = int(hipErrorOutOfMemory);
                 return;
             }
             item_capacity_ = max_items;
         }
         kernels::batch_count_items<index_type><<<detail::grid_for(num_tables_, 1, 8), block_size(), 0, stream>>>(
-            offsets, num_tables_, ik, items_);
+            begins, ends, order_, num_tables_, ik, items_);
         detail::device_scan(items_, first_item_, num_tables_, parts_, false, stream);
         kernels::batch_fill_items<index_type><<<detail::grid_for(num_tables_, 1, 8), block_size(), 0, stream>>>(
-            first_item_, offsets, num_tables_, ik, item_table_);
+            first_item_, begins, ends, order_, num_tables_, ik, item_table_, item_begin_);
         storage::kernels::fill<index_type><<<1, 64, 0, stream>>>(counter_, 1, index_type(0));
-        const unsigned grid = unsigned(detail::num_compute_units() * (launch.blocks_per_cu < 1 ? 1 : launch.blocks_per_cu));
-        if (launch.keys_per_batch <= 4)
-            launch_persistent<StatusHandler, (CGSize < 4 ? int(CGSize) : 4), IsInsert>(
-                grid, keys_in, values_in, values_out, offsets, ik, stream, probing_length, status_out);
-        else
-            launch_persistent<StatusHandler, (CGSize < 8 ? int(CGSize) : 8), IsInsert>(
-                grid, keys_in, values_in, values_out, offsets, ik, stream, probing_length, status_out);
+        const unsigned grid = unsigned(detail::num_compute_units() * launch.blocks_per_cu);
+        launch_owner<StatusHandler, IsInsert, -1>(single_config_, grid, unsigned(launch.block_size), keys_in,
+                                                  values_in, values_out, ends, ik, stream, probing_length,
+                                                  status_out);
         last_error_ = int(hipGetLastError());
     }
 
-    template<class StatusHandler, int U, bool IsInsert>
-    WC_H void launch_persistent(unsigned grid, const key_type* keys_in, const value_type* values_in,
-                                value_type* values_out, const index_type* offsets, index_type ik, hipStream_t stream,
-                                index_type probing_length, typename StatusHandler::base_type* status_out) noexcept {
-        kernels::batch_persistent<view_type, StatusHandler, U, IsInsert><<<grid, block_size(), 0, stream>>>(
-            keys_in, values_in, values_out, offsets, num_tables_, first_item_, item_table_, ik, counter_, tables_,
-            slots_, statuses_, seed_, probing_length, status_out);
+    // kernel for a single configuration `only` (or all, -1)
+    template<class StatusHandler, bool IsInsert, int I>
+    WC_H void launch_owner(int only, unsigned grid, unsigned block, const key_type* keys_in,
+                           const value_type* values_in, value_type* values_out, const index_type* ends,
+                           index_type ik, hipStream_t stream, index_type probing_length,
+                           typename StatusHandler::base_type* status_out) noexcept {
+        if constexpr (I < configs::count) {
+            if (only != I) {
+                launch_owner<StatusHandler, IsInsert, I + 1>(only, grid, block, keys_in, values_in, values_out, ends,
+                                                             ik, stream, probing_length, status_out);
+                return;
+            }
+        }
+        constexpr int Only = I < configs::count ? I : -1;
+        kernels::batch_owner<key_type, value_type, EmptyKey, TombstoneKey, StatusHandler, IsInsert, Only>
+            <<<grid, block, 0, stream>>>(keys_in, values_in, values_out, ends, items_, first_item_, num_tables_,
+                                         item_table_, item_begin_, ik, counter_, tables_, slots_, statuses_, seed_,
+                                         probing_length, status_out);
+    }
+
+    // insert + retrieve of the given ranges between two events; ms or < 0
+    WC_H double time_pass(const key_type* keys_in, const value_type* values_in, const index_type* begins,
+                          const index_type* ends, index_type num_in, const key_type* queries,
+                          const index_type* qbegins, const index_type* qends, index_type num_queries,
+                          value_type* values_out, hipStream_t stream, const launch_type& l, hipEvent_t e0,
+                          hipEvent_t e1) noexcept {
+        using SH = status_handlers::ReturnNothing;
+        init(stream);
+        (void)hipEventRecord(e0, stream);
+        run<SH, true>(keys_in, values_in, nullptr, begins, ends, num_in, stream, defaults::probing_length(),
+                      nullptr, l);
+        if (last_error_ != 0) return -1.0;
+        if (num_queries) {
+            run<SH, false>(queries, nullptr, values_out, qbegins, qends, num_queries, stream,
+                           defaults::probing_length(), nullptr, l);
+            if (last_error_ != 0) return -1.0;
+        }
+        (void)hipEventRecord(e1, stream);
+        if (hipEventSynchronize(e1) != hipSuccess) return -1.0;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return double(ms);
     }
 
     index_type num_tables_ = 0;
     key_type seed_;
-    index_type total_slots_ = 0;
+    std::vector<index_type> min_caps_;
     std::vector<index_type> capacities_;
+    std::vector<std::uint8_t> config_of_;
+    int fixed_config_;   // the configuration the batch was created with
+    int single_config_ = -1;
+    index_type total_slots_ = 0;
+    index_type arena_slots_ = 0;
+    launch_type launch_ = default_launch();
     bool ok_ = false;
     int last_error_ = 0;
     detail::BatchTableDesc* tables_ = nullptr;
     slot_type* slots_ = nullptr;
     Status* statuses_ = nullptr;
+    std::uint32_t* order_ = nullptr;
     index_type* parts_ = nullptr;
     index_type* counter_ = nullptr;
     index_type* items_ = nullptr;
     index_type* first_item_ = nullptr;
     std::uint32_t* item_table_ = nullptr;
+    index_type* item_begin_ = nullptr;
     index_type item_capacity_ = 0;
 };
+
+template<class Key, class Value, Key EmptyKey, Key TombstoneKey>
+WC_H int TableBatch<Key, Value, EmptyKey, TombstoneKey>::tune(const key_type* keys_in, const value_type* values_in,
+                                                             const index_type* offsets, index_type num_in,
+                                                             const key_type* queries,
+                                                             const index_type* query_offsets,
+                                                             index_type num_queries, value_type* values_out,
+                                                             hipStream_t stream, tune_report_type* report) noexcept {
+    if (!ok_) return int(hipErrorInvalidValue);
+    const index_type n = num_tables_;
+    std::vector<index_type> off(n + 1), qoff(n + 1, 0);
+    if (hipMemcpy(off.data(), offsets, sizeof(index_type) * (n + 1), hipMemcpyDeviceToHost) != hipSuccess)
+        return int(hipGetLastError());
+    if (num_queries &&
+        hipMemcpy(qoff.data(), query_offsets, sizeof(index_type) * (n + 1), hipMemcpyDeviceToHost) != hipSuccess)
+        return int(hipGetLastError());
+
+    // classes, and per class the end of every table's range (tables of
+    // other classes get empty ranges)
+    std::vector<std::uint32_t> cls(n);
+    std::vector<std::uint32_t> ids;
+    for (index_type t = 0; t < n; ++t) {
+        cls[t] = detail::batch_tune_class(min_caps_[t], off[t + 1] - off[t]);
+        ids.push_back(cls[t]);
+    }
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    const std::size_t nc = ids.size();
+    std::vector<index_type> ends_h(2 * nc * n);
+    for (std::size_t c = 0; c < nc; ++c)
+        for (index_type t = 0; t < n; ++t) {
+            const bool in = cls[t] == ids[c];
+            ends_h[(2 * c) * n + t] = in ? off[t + 1] : off[t];
+            ends_h[(2 * c + 1) * n + t] = in ? qoff[t + 1] : qoff[t];
+        }
+    index_type* ends_d = nullptr;
+    if (!malloc_(&ends_d, sizeof(index_type) * ends_h.size())) return int(hipErrorOutOfMemory);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = int(hipMemcpy(ends_d, ends_h.data(), sizeof(index_type) * ends_h.size(), hipMemcpyHostToDevice));
+    if (rc == 0 && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = int(hipErrorOutOfMemory);
+
+    const int fixed = fixed_config_;
+    const launch_type base = default_launch();
+    auto best_of = [&](const index_type* ends, const index_type* qends, index_type nk, index_type nq,
+                       const launch_type& l) {
+        double best = -1.0;
+        for (int rep = 0; rep < 3 && rc == 0; ++rep) {   // the first repetition warms up
+            const double ms = time_pass(keys_in, values_in, offsets, ends, nk, queries, query_offsets, qends, nq,
+                                        values_out, stream, l, e0, e1);
+            if (ms < 0.0) rc = last_error_ ? last_error_ : int(hipErrorUnknown);
+            else if (rep > 0 && (best < 0.0 || ms < best)) best = ms;
+        }
+        return best;
+    };
+
+    // per configuration: every class alone
+    std::vector<double> ms(configs::count * nc, -1.0);
+    std::vector<index_type> slots_of(configs::count * nc, 0);
+    std::vector<index_type> keys_of(nc, 0), queries_of(nc, 0), tables_of(nc, 0);
+    for (index_type t = 0; t < n; ++t) {
+        const std::size_t c = std::size_t(std::lower_bound(ids.begin(), ids.end(), cls[t]) - ids.begin());
+        keys_of[c] += off[t + 1] - off[t];
+        queries_of[c] += qoff[t + 1] - qoff[t];
+        tables_of[c] += 1;
+    }
+    double fixed_total = -1.0;
+    for (int k = 0; k < configs::count && rc == 0; ++k) {
+        if (!set_config(k)) {
+            rc = int(hipErrorOutOfMemory);
+            break;
+        }
+        for (index_type t = 0; t < n; ++t) {
+            const std::size_t c = std::size_t(std::lower_bound(ids.begin(), ids.end(), cls[t]) - ids.begin());
+            slots_of[k * nc + c] += capacities_[t];
+        }
+        for (std::size_t c = 0; c < nc && rc == 0; ++c)
+            ms[k * nc + c] = best_of(ends_d + (2 * c) * n, ends_d + (2 * c + 1) * n, keys_of[c],
+                                     num_queries ? queries_of[c] : 0, base);
+        if (k == fixed && rc == 0)
+            fixed_total = best_of(offsets + 1, query_offsets ? query_offsets + 1 : nullptr, num_in, num_queries, base);
+    }
+
+    // per class: the fastest configuration within 10% of the fewest slots
+    std::vector<int> chosen(nc, fixed);
+    for (std::size_t c = 0; c < nc && rc == 0; ++c) {
+        index_type fewest = ~index_type(0);
+        for (int k = 0; k < configs::count; ++k) fewest = std::min(fewest, slots_of[k * nc + c]);
+        double best = -1.0;
+        for (int k = 0; k < configs::count; ++k) {
+            if (double(slots_of[k * nc + c]) > 1.1 * double(fewest)) continue;
+            if (best < 0.0 || ms[k * nc + c] < best) {
+                best = ms[k * nc + c];
+                chosen[c] = k;
+            }
+        }
+    }
+
+    // chosen layout, then the launch candidates on the whole workload
+    double tuned_total = -1.0;
+    launch_type best_launch = base;
+    if (rc == 0) {
+        std::vector<std::uint8_t> per_table(n);
+        for (index_type t = 0; t < n; ++t)
+            per_table[t] = std::uint8_t(chosen[std::size_t(std::lower_bound(ids.begin(), ids.end(), cls[t]) -
+                                                           ids.begin())]);
+        if (!set_configs(per_table.data())) rc = int(hipErrorOutOfMemory);
+    }
+    const index_type item_choices[] = {512, 2048, 8192};
+    const int block_choices[] = {128, 256, 512};
+    const int waves_per_cu[] = {16, 32};
+    for (index_type ik : item_choices)
+        for (int bs : block_choices)
+            for (int wpc : waves_per_cu) {
+                if (rc != 0) break;
+                const launch_type l{ik, wpc * 64 / bs, bs};
+                const double t = best_of(offsets + 1, query_offsets ? query_offsets + 1 : nullptr, num_in,
+                                         num_queries, l);
+                if (rc == 0 && (tuned_total < 0.0 || t < tuned_total)) {
+                    tuned_total = t;
+                    best_launch = l;
+                }
+            }
+    if (rc == 0) {
+        launch_ = best_launch;
+        init(stream);
+        rc = int(hipStreamSynchronize(stream));
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(ends_d);
+    if (rc == 0 && report) {
+        report->classes.clear();
+        for (std::size_t c = 0; c < nc; ++c) {
+            detail::BatchTuneClass r;
+            r.id = ids[c];
+            r.tables = tables_of[c];
+            r.keys = keys_of[c];
+            r.queries = num_queries ? queries_of[c] : 0;
+            r.fixed_config = fixed;
+            r.chosen_config = chosen[c];
+            r.fixed_ms = ms[fixed * nc + c];
+            r.chosen_ms = ms[chosen[c] * nc + c];
+            r.fixed_slots = slots_of[fixed * nc + c];
+            r.chosen_slots = slots_of[chosen[c] * nc + c];
+            report->classes.push_back(r);
+        }
+        report->fixed_ms = fixed_total;
+        report->tuned_ms = tuned_total;
+        report->launch = launch_;
+    }
+    return rc;
+}
 
 }  // namespace warpcore

@@ -669,21 +669,70 @@ class TableBatch:
         _check(self._fn("retrieve")(self._h, _ptr(keys_in), _ptr(offsets), keys_in.numel(), _ptr(values_out),
                                     _stream(stream), probing_length, _ptr(status_out)), "retrieve")
 
-    def set_launch(self, keys_per_batch: int, item_keys: int, blocks_per_cu: int):
-        _check(self._fn("set_launch")(self._h, keys_per_batch, item_keys, blocks_per_cu), "set_launch")
+    def set_launch(self, item_keys: int, blocks_per_cu: int, block_size: int = 256):
+        _check(self._fn("set_launch")(self._h, item_keys, blocks_per_cu, block_size), "set_launch")
 
     def get_launch(self):
+        """(keys per work item, blocks per CU, threads per block)"""
         out = (ctypes.c_uint64 * 3)()
         _check(self._fn("get_launch")(self._h, out), "get_launch")
         return tuple(out)
 
     def tune(self, keys_in, values_in, offsets, queries, query_offsets, values_out, stream=None):
+        """Autotuner: per (capacity, fill) class configuration, then launch.
+        Returns (launch, tuned ms of the whole workload); leaves the batch
+        empty."""
         out = (ctypes.c_uint64 * 3)()
         ms = ctypes.c_double(0)
         _check(self._fn("tune")(self._h, _ptr(keys_in), _ptr(values_in), _ptr(offsets), keys_in.numel(),
                                 _ptr(queries), _ptr(query_offsets), queries.numel(), _ptr(values_out),
                                 _stream(stream), out, ctypes.byref(ms)), "tune")
         return tuple(out), ms.value
+
+    REPORT_FIELDS = ("class_id", "tables", "keys", "queries", "fixed_config", "chosen_config", "fixed_ns",
+                     "chosen_ns", "fixed_slots", "chosen_slots")
+
+    def tune_report(self):
+        """Per class of the last tune: dicts of REPORT_FIELDS plus the
+        (lanes, window) of both configurations; and the whole-workload
+        (fixed ms, tuned ms)."""
+        n = self._fn("tune_report")(self._h, None, 0, None)
+        _check(0 if n >= 0 else n, "tune_report")
+        rows = (ctypes.c_uint64 * (10 * max(n, 1)))()
+        totals = (ctypes.c_double * 2)()
+        _check(0 if self._fn("tune_report")(self._h, rows, n, totals) >= 0 else -2, "tune_report")
+        shapes = self.configs()
+        out = []
+        for i in range(n):
+            r = dict(zip(self.REPORT_FIELDS, rows[10 * i: 10 * i + 10]))
+            r["capacity_band"], r["fill_band"] = r["class_id"] // 4, r["class_id"] % 4
+            r["fixed_shape"], r["chosen_shape"] = shapes[r["fixed_config"]], shapes[r["chosen_config"]]
+            out.append(r)
+        return out, (totals[0], totals[1])
+
+    def configs(self):
+        """(lanes, window) of every configuration index"""
+        out = []
+        for c in range(self._fn("num_configs")()):
+            lw = (ctypes.c_uint32 * 2)()
+            _check(self._fn("config_shape")(c, lw), "config_shape")
+            out.append((lw[0], lw[1]))
+        return out
+
+    def set_configs(self, configs):
+        """Per-table configuration indices; lays the tables out again and
+        empties the batch."""
+        import numpy as np
+        c = np.ascontiguousarray(configs, dtype=np.uint8)
+        _check(self._fn("set_configs")(self._h, c.ctypes.data_as(ctypes.c_void_p), len(c)), "set_configs")
+
+    def table_config(self, table: int) -> int:
+        c = self._fn("table_config")(self._h, table)
+        _check(0 if c >= 0 else c, "table_config")
+        return c
+
+    def window(self, table: int) -> int:
+        return self.configs()[self.table_config(table)][1]
 
     def status_words(self, stream=None):
         out = torch.empty(self.num_tables(), dtype=torch.int32, device="cuda")
