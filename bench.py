@@ -279,15 +279,16 @@ def run_batched(g: Gpu, num_tables: int = 65536, steps: int = 3, warmup: int = 1
     # plain launches: one SingleValueHashTable per table for a sample of tables
     pick = np.linspace(0, num_tables - 1, sample_tables).astype(np.int64)
     tables = [SingleValueHashTable(int(caps[t]), 32, 32, cg_size=16) for t in pick]
+    # views cut before timing, so the timed loop is the launches themselves
+    parts = [(keys[int(off[t]):int(off[t + 1])], values[int(off[t]):int(off[t + 1])], out[int(off[t]):int(off[t + 1])])
+             for t in pick]
+    sample_keys = sum(int(off[t + 1] - off[t]) for t in pick)
     g.stream.synchronize()
     e0, e1 = g.event(), g.event()
     e0.record(g.stream)
-    sample_keys = 0
-    for t, tab in zip(pick, tables):
-        lo, hi = int(off[t]), int(off[t + 1])
-        tab.insert(keys[lo:hi], values[lo:hi], stream=g.stream)
-        tab.retrieve(keys[lo:hi], out[lo:hi], stream=g.stream)
-        sample_keys += hi - lo
+    for tab, (k, v, o) in zip(tables, parts):
+        tab.insert(k, v, stream=g.stream)
+        tab.retrieve(k, o, stream=g.stream)
     e1.record(g.stream)
     g.stream.synchronize()
     plain_ms = e0.elapsed_time(e1)
@@ -337,9 +338,10 @@ def run_serving(g: Gpu, num_tables: int = 64, total_bytes: float = 100e9, num_re
     del hot, take
     res = {"tables": num_tables, "table_slots": int(caps[0]), "requests": num_requests, "keys": n}
 
-    def one(mode, arrival, count=num_requests):
+    def one(mode, arrival, count=num_requests, pick=None):
         srv = Server(caps, 64, 64, **(server_kw or {}))
-        lat, mk = srv.drive(mode, ops[:count], tabs[:count], sizes[:count], arrival[:count], keys, values, out, st)
+        sel = slice(0, count) if pick is None else pick
+        lat, mk = srv.drive(mode, ops[sel], tabs[sel], sizes[sel], arrival[sel], keys, values, out, st)
         stats = srv.stats()
         steps = (stats["steps"], stats["direct_steps"]) if mode == 0 else (count, 0)
         del srv
@@ -355,6 +357,21 @@ def run_serving(g: Gpu, num_tables: int = 64, total_bytes: float = 100e9, num_re
     res["throughput"] = {"server_gkeys_s": round(n / mk_s / 1e3, 3), "launch_per_request_gkeys_s":
                          round(n / mk_b / 1e3, 3), "speedup": round(mk_b / mk_s, 3), "server_steps": steps_s[0],
                          "server_direct_steps": steps_s[1]}
+    # throughput per request-size class: the class's requests alone, all
+    # at t=0, through the server and one launch per request
+    per_class = []
+    for lo, hi in [(8, 12), (12, 16), (16, 19), (19, 22.01)]:
+        pick = np.flatnonzero((sizes >= 2 ** lo) & (sizes < 2 ** hi))
+        if len(pick) == 0:
+            continue
+        kc = int(sizes[pick].sum())
+        _, mk_cb, _ = one(1, zero, pick=pick)
+        _, mk_cs, _ = one(0, zero, pick=pick)
+        per_class.append({"sizes": f"2^{lo}..2^{int(hi)}", "requests": int(len(pick)), "keys": kc,
+                          "key_share": round(kc / n, 4), "server_gkeys_s": round(kc / mk_cs / 1e3, 3),
+                          "launch_per_request_gkeys_s": round(kc / mk_cb / 1e3, 3),
+                          "speedup": round(mk_cb / mk_cs, 2)})
+    res["throughput"]["per_size_class"] = per_class
     rate = load * num_requests / mk_b          # requests per microsecond
     arrival = np.cumsum(rng.exponential(1.0 / rate, size=num_requests))
     lat_b, mk_b2, _ = one(1, arrival)

@@ -623,8 +623,10 @@ WC_API uint32_t wc_batch_u64u64_cg_size(void* batch);
  * insert that does not fit even then is refused: every key reports
  * out_of_memory and wait returns 0.
  *   create       table_capacities: HOST array (minimum capacities);
- *                max_requests per step (0 = 1024), item_keys per work item
- *                (0 = 64), max_step_keys (0 = 2^24), max_load (<= 0: 0.9),
+ *                max_requests per step (0 = 1024), item_keys: largest work
+ *                item of the step kernel (0 = 2048; a step uses the smallest
+ *                power of two from 64 that gives every wavefront about four
+ *                items), max_step_keys (0 = 2^24), max_load (<= 0: 0.9),
  *                blocks_per_cu (0 = 8), direct_keys (0 = 2^18, UINT64_MAX =
  *                every step on the step kernel)
  *   submit       device buffers, kept valid until the request is done;

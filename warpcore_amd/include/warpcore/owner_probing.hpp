@@ -260,6 +260,12 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
 // Lookups of keys_in[first .. end) by the calling wavefront. A hit is
 // written by the lane holding the slot; the owner writes the status.
 //
+// Erase (single_value_hash_table.cuh:425-468 per key): the same walk; at a
+// hit the lane holding the slot writes TOMBSTONE over the key and the key
+// reports none; key_not_found and probing_length_exceeded are reported but
+// not joined into the table status (invalid keys are); values_out is not
+// used.
+//
 // Overflow filter (cores with one, see mark_overflow): a key whose window
 // is full and holds no copy of it reads the window's filter byte (next
 // round, from the MALL-sized filter instead of another window) and ends as
@@ -269,7 +275,7 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
 // cycle of windows and the wavefront has seen a window with an EMPTY slot
 // (the table is static during the call), where the walk would end at an
 // EMPTY slot with key_not_found.
-template<class Core, class StatusHandler, class W, unsigned SPL = 1>
+template<class Core, class StatusHandler, class W, unsigned SPL = 1, bool Erase = false>
 WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_in, index_t first, index_t end,
                                typename Core::out_type* __restrict__ values_out, const Core& core, index_t budget,
                                typename StatusHandler::base_type* __restrict__ status_out,
@@ -304,7 +310,7 @@ WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_
         return;
     }
     O miss_value{};
-    const Status miss_status = core.missing(miss_value);
+    const Status miss_status = Erase ? Status::key_not_found() : core.missing(miss_value);
 
     K key = K{};
     W w = 0, step = 1;
@@ -376,7 +382,10 @@ WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_
             for (unsigned x = 0; x < SPL; ++x) {
                 const K tk = Core::key_of(s[j][x]);
                 const bool h = on && tk == e.key;
-                if (h) values_out[first + e.pos] = core.found(Core::value_of(s[j][x]));   // the slot's lane writes the hit
+                if (h) {   // the slot's lane writes the hit
+                    if constexpr (Erase) core.erase_slot(index_t(e.w) * WIN + rank * SPL + x);
+                    else values_out[first + e.pos] = core.found(Core::value_of(s[j][x]));
+                }
                 const std::uint64_t hm = ballot(h);
                 const std::uint64_t em = ballot(on && Core::is_empty_key(tk));
                 any_empty |= em;
@@ -388,9 +397,13 @@ WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_
         }
         seen_empty = seen_empty || any_empty != 0;
         auto miss = [&]() {
-            join(miss_status);
-            StatusHandler::handle(miss_status, status_out, pos);
-            if (!miss_status.has_any()) values_out[pos] = miss_value;
+            if constexpr (Erase) {
+                StatusHandler::handle(Status::key_not_found(), status_out, pos);
+            } else {
+                join(miss_status);
+                StatusHandler::handle(miss_status, status_out, pos);
+                if (!miss_status.has_any()) values_out[pos] = miss_value;
+            }
             phase = kFree;
         };
         auto next_window = [&]() {
@@ -406,7 +419,7 @@ WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_
             } else if (empty) {
                 miss();
             } else if (rnd + 1 >= budget) {
-                join(Status::probing_length_exceeded());
+                if constexpr (!Erase) join(Status::probing_length_exceeded());
                 StatusHandler::handle(Status::probing_length_exceeded(), status_out, pos);
                 phase = kFree;
             } else if (filter != nullptr && seen_empty) {
@@ -464,6 +477,24 @@ __global__ __launch_bounds__(256) void retrieve_owner(const typename Core::key_t
                                                                   core.geometry().windows);
     detail::retrieve_owner_range<Core, StatusHandler, W>(keys_in, first, end, values_out, core, budget, status_out,
                                                          lds + (threadIdx.x & ~63u), join);
+    join.flush(core.status_word());
+}
+
+template<class Core, class StatusHandler, class W>
+__global__ __launch_bounds__(256) void erase_owner(const typename Core::key_type* __restrict__ keys_in,
+                                                   index_t num_in, Core core, index_t probing_length,
+                                                   typename StatusHandler::base_type* __restrict__ status_out) {
+    __shared__ detail::OwnerEntry<typename Core::key_type, W> lds[256];
+    detail::DeferredJoin join;
+    const index_t wave = (index_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+    const index_t waves = (index_t(gridDim.x) * blockDim.x) / 64;
+    const index_t per = SDIV(num_in, waves);
+    const index_t first = wave * per < num_in ? wave * per : num_in;
+    const index_t end = first + per < num_in ? first + per : num_in;
+    const index_t budget = probing_schemes::detail::window_budget(probing_length, Core::cg_size(),
+                                                                  core.geometry().windows);
+    detail::retrieve_owner_range<Core, StatusHandler, W, 1, true>(keys_in, first, end, nullptr, core, budget,
+                                                                  status_out, lds + (threadIdx.x & ~63u), join);
     join.flush(core.status_word());
 }
 

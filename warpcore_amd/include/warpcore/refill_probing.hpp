@@ -766,6 +766,19 @@ WC_H void launch_retrieve_owner(const typename Core::key_type* keys_in, index_t 
     }
 }
 
+template<class Core, class StatusHandler>
+WC_H void launch_erase_owner(const typename Core::key_type* keys_in, index_t num_in, const Core& core,
+                             index_t probing_length, typename StatusHandler::base_type* status_out,
+                             hipStream_t stream) noexcept {
+    if (core.geometry().windows < (index_t(1) << 31)) {
+        constexpr auto k = kernels::erase_owner<Core, StatusHandler, std::uint32_t>;
+        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, num_in, core, probing_length, status_out);
+    } else {
+        constexpr auto k = kernels::erase_owner<Core, StatusHandler, index_t>;
+        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, num_in, core, probing_length, status_out);
+    }
+}
+
 // Lane-owner kernels apply to single-value cores whose windows are 2..16
 // slots (a 128-byte line of 8- or 16-byte slots at most: a group reads one
 // window per owner, so wider windows would hold 32 or 64 windows per lane).

@@ -139,12 +139,13 @@ __global__ __launch_bounds__(256) void serve_prepare(const Desc* __restrict__ st
 #define WC_SERVE_OCCUPANCY
 #endif
 
-// One step: group g of the (resident) grid takes work items g, g + G, ...
-// (items hold item_keys keys, so a static assignment balances and needs no
-// shared counter). An item's request is found by binary search in an LDS
-// copy of the requests' first items. Lookups and inserts use the refill
-// kernels' range code, erases the lockstep one.
-template<class View, int UR, int UI, int UE>
+// One step: wavefront v of the (resident) grid takes work items v, v + V,
+// ... (items hold item_keys keys, so a static assignment balances and
+// needs no shared counter). An item's request is found by binary search in
+// an LDS copy of the requests' first items; the wavefront then runs the
+// lane-owner range of the request's operation (owner_probing.hpp) on the
+// request's table, 64 keys in flight.
+template<class View>
 __global__ __launch_bounds__(256) WC_SERVE_OCCUPANCY void serve_step(const serving::RequestDesc<typename View::key_type,
                                                                              typename View::value_type>* __restrict__ reqs,
                                                   const index_t* __restrict__ first_item,
@@ -152,50 +153,42 @@ __global__ __launch_bounds__(256) WC_SERVE_OCCUPANCY void serve_step(const servi
                                                   const View* __restrict__ views, index_t probing_length) {
     using K = typename View::key_type;
     using W = std::uint32_t;
-    constexpr unsigned CG = unsigned(View::cg_size());
-    __shared__ detail::StagedKey<K, W> stage[256 / CG * UE];
-    __shared__ detail::RefillEntry<K, W> refill[256];
-    __shared__ detail::RefillInsertEntry<K, W, typename View::value_type> refill_insert[256];
+    static_assert(detail::owner_capable<View>(), "the step kernel runs the lane-owner ranges");
+    __shared__ detail::OwnerEntry<K, W> lds[256];
     extern __shared__ index_t first_lds[];
-    auto* my = stage + (threadIdx.x / CG) * UE;
-    auto* my_refill = refill + (threadIdx.x & ~(CG - 1));
-    auto* my_refill_insert = refill_insert + (threadIdx.x & ~(CG - 1));
-    const unsigned rank = lane_id() & (CG - 1);
+    auto* my = lds + (threadIdx.x & ~63u);
     const index_t num = state[0] < max_reqs ? state[0] : max_reqs;
     const index_t total = state[1];
     item_keys = state[2];
     for (index_t r = threadIdx.x; r < num; r += blockDim.x) first_lds[r] = first_item[r];
     __syncthreads();
     using H = status_handlers::ReturnStatus;
-    const index_t gid = (index_t(blockIdx.x) * blockDim.x + threadIdx.x) / CG;
-    const index_t groups = (index_t(gridDim.x) * blockDim.x) / CG;
-    for (index_t item = gid; item < total; item += groups) {
-        {
-            index_t lo = 0, hi = num;   // last request whose first item is <= item
-            while (hi - lo > 1) {
-                const index_t mid = (lo + hi) / 2;
-                if (first_lds[mid] <= item) lo = mid;
-                else hi = mid;
-            }
-            const std::uint32_t r = std::uint32_t(lo);
-            const auto d = reqs[r];
-            const index_t begin = (item - first_lds[r]) * item_keys;
-            const index_t end = begin + item_keys < d.n ? begin + item_keys : d.n;
-            const View view = views[d.table];
-            const index_t budget = probing_schemes::detail::window_budget(probing_length, CG, view.geo.windows);
-            detail::DeferredJoin join;
-            auto* st = reinterpret_cast<Status*>(d.status_out);
-            if (d.op == std::uint32_t(serving::Op::insert))
-                detail::insert_refill_range<View, H, UI, W, true>(d.keys, d.values, begin, end, view, budget, st,
-                                                                  my_refill_insert, join);
-            else if (d.op == std::uint32_t(serving::Op::retrieve))
-                detail::retrieve_refill_range<View, H, UR, W>(d.keys, begin, end, d.values_out, view, budget, st,
-                                                              my_refill, join);
-            else
-                detail::erase_lockstep_range<View, H, UE, W>(d.keys, begin, end, UE, view, budget, st, my, join);
-            const status_base_t all = tile<CG>::reduce_or(join.acc);
-            if (rank == 0 && all != 0) view.status->atomic_join(Status::from_base(all));
+    const index_t wave = (index_t(blockIdx.x) * blockDim.x + threadIdx.x) / 64;
+    const index_t waves = (index_t(gridDim.x) * blockDim.x) / 64;
+    for (index_t item = wave; item < total; item += waves) {
+        index_t lo = 0, hi = num;   // last request whose first item is <= item
+        while (hi - lo > 1) {
+            const index_t mid = (lo + hi) / 2;
+            if (first_lds[mid] <= item) lo = mid;
+            else hi = mid;
         }
+        const std::uint32_t r = std::uint32_t(lo);
+        const auto d = reqs[r];
+        const index_t begin = (item - first_lds[r]) * item_keys;
+        const index_t end = begin + item_keys < d.n ? begin + item_keys : d.n;
+        const View view = views[d.table];
+        const index_t budget = probing_schemes::detail::window_budget(probing_length, View::cg_size(),
+                                                                      view.geo.windows);
+        detail::DeferredJoin join;
+        auto* st = reinterpret_cast<Status*>(d.status_out);
+        if (d.op == std::uint32_t(serving::Op::insert))
+            detail::insert_owner_range<View, H, W, true>(d.keys, d.values, begin, end, view, budget, st, my, join);
+        else if (d.op == std::uint32_t(serving::Op::retrieve))
+            detail::retrieve_owner_range<View, H, W>(d.keys, begin, end, d.values_out, view, budget, st, my, join);
+        else
+            detail::retrieve_owner_range<View, H, W, 1, true>(d.keys, begin, end, nullptr, view, budget, st, my,
+                                                              join);
+        join.flush(view.status);
     }
 }
 
@@ -228,7 +221,7 @@ public:
 
     struct Config {
         index_t max_requests = 1024;                 // requests (chunks) per step: descriptor array
-        index_t item_keys = 64;                      // keys per work item
+        index_t item_keys = 2048;                    // keys per work item (at most)
         index_t max_step_keys = index_t(1) << 24;    // keys per step
         float max_load = 0.9f;                       // admission: slot budget per table
         int blocks_per_cu = 8;                       // persistent grid
@@ -628,10 +621,7 @@ private:
         return hipGetLastError();
     }
 
-    WC_H static auto step_kernel() noexcept {
-        return &kernels::serve_step<view_type, detail::refill_keys<unsigned(CG)>(),
-                                    detail::refill_insert_keys<unsigned(CG)>(), detail::lockstep_keys<unsigned(CG)>()>;
-    }
+    WC_H static auto step_kernel() noexcept { return &kernels::serve_step<view_type>; }
 
     // The step as a HIP graph built node by node (no stream capture, so no
     // other thread's device calls can meet a capture in progress): the
@@ -710,17 +700,18 @@ private:
         return hipGraphLaunch(graphs_[b].exec, stream_);
     }
 
-    // Keys per work item of a step: the smallest power of two from 8 that
-    // still gives every group about four items (at most cfg_.item_keys).
-    // A group takes ceil(items / groups) items, so small items keep the
-    // busiest group close to the average and a small step ends sooner;
-    // large steps keep large items, whose per-item setup is amortised.
+    // Keys per work item of a step: the smallest power of two from 64 (one
+    // key per lane) that still gives every wavefront about four items (at
+    // most cfg_.item_keys). A wavefront takes ceil(items / wavefronts)
+    // items, so small items keep the busiest wavefront close to the average
+    // and a small step ends sooner; large steps keep large items, whose
+    // per-item setup and drain are amortised.
     index_t step_item_keys(const StepList& step) const noexcept {
         index_t keys = 0;
         for (const auto& c : step) keys += c.keys;
-        const index_t groups = index_t(step_grid_) * block_size() / CG;
-        index_t ik = 8;
-        while (ik < cfg_.item_keys && ik * groups * 4 < keys) ik *= 2;
+        const index_t waves = index_t(step_grid_) * block_size() / 64;
+        index_t ik = 64;
+        while (ik < cfg_.item_keys && ik * waves * 4 < keys) ik *= 2;
         return ik < cfg_.item_keys ? ik : cfg_.item_keys;
     }
 

@@ -576,6 +576,13 @@ public:
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!is_initialized_ || num_in == 0) return;
         if constexpr (!soa_storage && detail::fixed_step_scheme<typename view_type::probing_scheme_type>::value) {
+            if constexpr (detail::owner_capable<view_type>()) {
+                if (detail::owner_lookups()) {
+                    detail::launch_erase_owner<view_type, StatusHandler>(keys_in, num_in, view(), probing_length,
+                                                                         status_out, stream);
+                    return;
+                }
+            }
             if (detail::lockstep_lookups()) {
                 detail::launch_erase_lockstep<view_type, StatusHandler, detail::lockstep_keys<unsigned(cg_size())>()>(
                     keys_in, num_in, view(), probing_length, status_out, stream);
