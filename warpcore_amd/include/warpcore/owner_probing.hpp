@@ -192,6 +192,13 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const unsigned ph = phase;   // this round's phase (the claim outcome below may change it)
+        // claiming owners: CAS on the lowest untried EMPTY slot, issued ahead
+        // of the window loads, so that its outcome (and the value store that
+        // follows a claim) waits for the CAS alone
+        const unsigned lead = owner_slot_of_bit<CG, SPL>(unsigned(ffs64(std::uint64_t(emp)) - 1));
+        typename Core::cas_word old{};
+        if (ph == kClaim) old = core.cas_issue(slots + index_t(w) * WIN + lead, seen, key, val);
         // only the loaded words stay in registers while the loads are in
         // flight; the owners' entries are read again from LDS to compare
         Probe probe[CG][SPL];
@@ -202,10 +209,19 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
             if constexpr (SPL == 1) probe[j][0] = Core::load_insert_probe(slots + index_t(at) * WIN + rank);
             else owner_load_pair<SPL>(slots + index_t(at) * WIN + rank * SPL, probe[j]);
         }
-        // claiming owners: CAS on the lowest untried EMPTY slot
-        const unsigned lead = owner_slot_of_bit<CG, SPL>(unsigned(ffs64(std::uint64_t(emp)) - 1));
-        typename Core::cas_word old{};
-        if (phase == kClaim) old = core.cas_issue(slots + index_t(w) * WIN + lead, seen, key, val);
+        if (ph == kClaim) {
+            const int code = core.cas_outcome(old, seen, key);
+            if (code == 1 || code == 2) {   // claimed, or the key was already there
+                Status st = code == 1 ? Status::none() : Status::duplicate_key();
+                core.finish_insert(index_t(w) * WIN + lead, val, code == 1 && Core::publishes_value, st, join);
+                if (code == 2) join(Status::duplicate_key());
+                StatusHandler::handle(st, status_out, pos);
+                phase = kFree;
+            } else if (code == 0) {   // another key took the slot: the window's next EMPTY slot
+                emp &= Bits(emp - 1);
+                if (emp == 0) advance();
+            }                         // code 3: the same slot again, with its refreshed word
+        }
         __builtin_amdgcn_wave_barrier();
 
         // each window's hits and EMPTY slots, extracted by its owner
@@ -226,7 +242,7 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
             }
         }
 
-        if (phase == kLoad) {
+        if (ph == kLoad) {
             if (hit != 0) {   // duplicate: the value is overwritten in place
                 Status st = Status::duplicate_key();
                 core.finish_insert(index_t(w) * WIN + owner_slot_of_bit<CG, SPL>(unsigned(ffs64(std::uint64_t(hit)) - 1)),
@@ -241,18 +257,6 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
             } else {
                 advance();
             }
-        } else if (phase == kClaim) {
-            const int code = core.cas_outcome(old, seen, key);
-            if (code == 1 || code == 2) {   // claimed, or the key was already there
-                Status st = code == 1 ? Status::none() : Status::duplicate_key();
-                core.finish_insert(index_t(w) * WIN + lead, val, code == 1 && Core::publishes_value, st, join);
-                if (code == 2) join(Status::duplicate_key());
-                StatusHandler::handle(st, status_out, pos);
-                phase = kFree;
-            } else if (code == 0) {   // another key took the lane: the window's next EMPTY lane
-                emp &= Bits(emp - 1);
-                if (emp == 0) advance();
-            }                         // code 3: the same lane again, with its refreshed word
         }
     }
 }
