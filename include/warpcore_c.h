@@ -261,7 +261,7 @@ WC_API uint32_t wc_counting_u64_cg_size(void* table);
  * counts, per-occurrence statuses (which occurrence of a new key reports
  * `none` is unspecified, as in _insert) and table status as
  * wc_counting_u64_insert, with the combining buffers in a fixed device array
- * of the library (28 MiB per GPU, held until the process ends) instead of
+ * of the library (30 MiB per GPU, held until the process ends) instead of
  * memory the table allocates, and no host synchronisation. It calls no
  * hipMalloc/hipFree and no allocating library routine.
  *   num_in at most wc_counting_u64_insert_noalloc_max() (2^20); larger

@@ -146,6 +146,11 @@ WC_H void set_counting_combine(int m) noexcept { counting_combine_setting().stor
 
 constexpr unsigned kCombineTile = 1024;             // occurrences per block and tile
 constexpr unsigned kCombineSlots = 2 * kCombineTile;  // LDS hash slots (load <= 1/2)
+constexpr unsigned kCombineHot = 512;                 // block-persistent map of keys frequent in a tile
+constexpr unsigned kCombineHotMin = 2;                // occurrences in a tile that make a key frequent
+constexpr unsigned kCombineHotProbes = 8;             // slots a key tries in it before it is emitted per tile
+constexpr unsigned kCombineBlocks = 1024;             // largest grid of count_combine
+constexpr std::uint32_t kHotEntry = 0x80000000u;      // pos[i] flag: entry via hot_entry[pos & ~flag]
 
 }  // namespace detail
 
@@ -153,19 +158,37 @@ namespace kernels {
 
 // One tile of 1024 occurrences per block: identical keys are combined in an
 // LDS hash map (CAS on the key, atomicAdd on the count, atomicMin on the
-// first occurrence), then the distinct keys of the tile are appended to
-// keys_out / counts_out with rep_out = their first occurrence, and every
-// occurrence gets pos[i] = its entry. Invalid keys (EMPTY, TOMBSTONE) get an
-// entry each, so the insert reports them.
+// first occurrence). A key that occurs kCombineHotMin times or more in the
+// tile moves to a second LDS map that lives for all the block's tiles
+// (kCombineHot slots, kCombineHotProbes tries), so a key that is frequent in the whole input
+// becomes one entry per block instead of one per tile (the entries' count
+// adds to the table are memory-side atomics, and those on one slot
+// serialise). The other distinct keys of the tile are appended to keys_out
+// / counts_out with rep_out = their first occurrence, and every occurrence
+// gets pos[i] = its entry, or kHotEntry | its hot slot, whose entry the
+// block writes to hot_entry when it flushes the second map at its end.
+// Invalid keys (EMPTY, TOMBSTONE) get an entry each, so the insert reports
+// them. Grid: at most kCombineBlocks blocks.
 template<class K, class V, K EmptyKey, K TombstoneKey>
 __global__ __launch_bounds__(256) void count_combine(const K* __restrict__ keys_in, index_t num_in,
                                                      K* __restrict__ keys_out, V* __restrict__ counts_out,
                                                      std::uint32_t* __restrict__ rep_out,
-                                                     std::uint32_t* __restrict__ pos, index_t* __restrict__ num_out) {
+                                                     std::uint32_t* __restrict__ pos, index_t* __restrict__ num_out,
+                                                     std::uint32_t* __restrict__ hot_entry) {
     constexpr unsigned T = detail::kCombineTile, S = detail::kCombineSlots, P = T / 256, Q = S / 256;
+    constexpr unsigned H = detail::kCombineHot, QH = H / 256;
+    static_assert(QH >= 1 && (H & (H - 1)) == 0, "hot map: a power of two of at least 256 slots");
     __shared__ K lkey[S];
     __shared__ std::uint32_t lcnt[S], lrep[S], lent[S];
+    __shared__ K hkey[H];
+    __shared__ std::uint32_t hcnt[H], hrep[H];
     __shared__ index_t base_sh;
+    for (unsigned q = 0; q < QH; ++q) {
+        const unsigned h = q * 256 + threadIdx.x;
+        hkey[h] = EmptyKey;
+        hcnt[h] = 0;
+        hrep[h] = ~0u;
+    }
     for (index_t tile = index_t(blockIdx.x) * T; tile < num_in; tile += index_t(gridDim.x) * T) {
         for (unsigned q = 0; q < Q; ++q) {
             const unsigned s = q * 256 + threadIdx.x;
@@ -199,9 +222,31 @@ __global__ __launch_bounds__(256) void count_combine(const K* __restrict__ keys_
             slot[j] = h;
         }
         __syncthreads();
-        // entries of this thread: its Q LDS slots that hold a key, then its invalid keys
-        unsigned mine = invalid;
-        for (unsigned q = 0; q < Q; ++q) mine += lkey[threadIdx.x * Q + q] != EmptyKey;
+        // keys repeated in this tile join the block's hot map (their tile
+        // count goes there; lcnt 0 marks the slot as moved)
+        unsigned mine = invalid;   // entries of this thread: its un-moved LDS slots, then its invalid keys
+        for (unsigned q = 0; q < Q; ++q) {
+            const unsigned s = threadIdx.x * Q + q;
+            const K k = lkey[s];
+            if (k == EmptyKey) continue;
+            bool moved = false;
+            if (lcnt[s] >= detail::kCombineHotMin) {
+                unsigned h = unsigned(hashers::MurmurHash<K>::hash(k)) & (H - 1);
+                for (unsigned t = 0; t < detail::kCombineHotProbes; ++t) {
+                    const K prev = detail::atomic_cas(hkey + h, EmptyKey, k);
+                    if (prev == EmptyKey || prev == k) {
+                        atomicAdd(hcnt + h, lcnt[s]);
+                        atomicMin(hrep + h, std::uint32_t(tile + lrep[s]));
+                        lent[s] = detail::kHotEntry | (std::uint32_t(blockIdx.x) * H + h);
+                        lcnt[s] = 0;
+                        moved = true;
+                        break;
+                    }
+                    h = (h + 1) & (H - 1);
+                }
+            }
+            mine += moved ? 0u : 1u;
+        }
         index_t total;
         const index_t incl = detail::block_inclusive_scan(index_t(mine), total);
         if (threadIdx.x == 0) base_sh = detail::atomic_add(num_out, total);
@@ -209,7 +254,7 @@ __global__ __launch_bounds__(256) void count_combine(const K* __restrict__ keys_
         index_t e = base_sh + incl - mine;
         for (unsigned q = 0; q < Q; ++q) {
             const unsigned s = threadIdx.x * Q + q;
-            if (lkey[s] != EmptyKey) {
+            if (lkey[s] != EmptyKey && lcnt[s] != 0) {
                 keys_out[e] = lkey[s];
                 counts_out[e] = V(lcnt[s]);
                 rep_out[e] = std::uint32_t(tile + lrep[s]);
@@ -232,6 +277,25 @@ __global__ __launch_bounds__(256) void count_combine(const K* __restrict__ keys_
         for (unsigned j = 0; j < P; ++j)
             if (slot[j] < S) pos[tile + j * 256 + threadIdx.x] = lent[slot[j]];
         __syncthreads();
+    }
+    // flush the hot map: one entry per key, its slot's entry to hot_entry
+    unsigned mine = 0;
+    for (unsigned q = 0; q < QH; ++q) mine += hkey[threadIdx.x * QH + q] != EmptyKey;
+    index_t total;
+    const index_t incl = detail::block_inclusive_scan(index_t(mine), total);
+    if (threadIdx.x == 0 && total != 0) base_sh = detail::atomic_add(num_out, total);
+    __syncthreads();
+    if (total == 0) return;
+    index_t e = base_sh + incl - mine;
+    for (unsigned q = 0; q < QH; ++q) {
+        const unsigned h = threadIdx.x * QH + q;
+        if (hkey[h] != EmptyKey) {
+            keys_out[e] = hkey[h];
+            counts_out[e] = V(hcnt[h]);
+            rep_out[e] = hrep[h];
+            hot_entry[std::uint32_t(blockIdx.x) * H + h] = std::uint32_t(e);
+            ++e;
+        }
     }
 }
 
@@ -258,13 +322,15 @@ __global__ __launch_bounds__(256) void count_join_duplicates(const V* __restrict
 // duplicate), else the entry's status (the table was full, ...).
 template<class StatusHandler>
 __global__ __launch_bounds__(256) void count_expand(const std::uint32_t* __restrict__ pos,
+                                                    const std::uint32_t* __restrict__ hot_entry,
                                                     const std::uint32_t* __restrict__ rep,
                                                     const Status* __restrict__ entry_status, index_t first,
                                                     index_t num_in,
                                                     typename StatusHandler::base_type* __restrict__ status_out) {
     const index_t stride = index_t(gridDim.x) * blockDim.x;
     for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < num_in; i += stride) {
-        const std::uint32_t e = pos[i];
+        const std::uint32_t p = pos[i];
+        const std::uint32_t e = (p & detail::kHotEntry) ? hot_entry[p & ~detail::kHotEntry] : p;
         Status s = entry_status[e];
         if (rep[e] != std::uint32_t(i)) {
             const Status stored = s - Status::index_overflow();
@@ -402,14 +468,18 @@ public:
         std::uint32_t* rep = nullptr;
         std::uint32_t* pos = nullptr;
         Status* status = nullptr;
+        std::uint32_t* hot = nullptr;   // entry of each block's hot-map slot
         index_type* num = nullptr;
         index_type capacity = 0;
 
         WC_HD static constexpr index_type align(index_type b) noexcept { return (b + 255) / 256 * 256; }
+        WC_HD static constexpr index_type hot_bytes() noexcept {
+            return align(sizeof(std::uint32_t) * detail::kCombineBlocks * detail::kCombineHot);
+        }
         // bytes of a buffer for `n` occurrences
         WC_HD static constexpr index_type bytes_for(index_type n) noexcept {
             return align(sizeof(key_type) * n) + align(sizeof(value_type) * n) + 3 * align(sizeof(std::uint32_t) * n) +
-                   align(sizeof(index_type));
+                   hot_bytes() + align(sizeof(index_type));
         }
         // the buffers inside `base` (256-byte aligned, bytes_for(n) bytes)
         WC_HD static CombineBuffers carve(void* base, index_type n) noexcept {
@@ -425,6 +495,8 @@ public:
             p += align(sizeof(std::uint32_t) * n);
             b.status = reinterpret_cast<Status*>(p);
             p += align(sizeof(std::uint32_t) * n);
+            b.hot = reinterpret_cast<std::uint32_t*>(p);
+            p += hot_bytes();
             b.num = reinterpret_cast<index_type*>(p);
             b.capacity = n;
             return b;
@@ -436,7 +508,7 @@ public:
     template<class StatusHandler>
     WC_H void insert_combined(const key_type* keys_in, index_type num_in, hipStream_t stream,
                               index_type probing_length, typename StatusHandler::base_type* status_out) noexcept {
-        constexpr index_type chunk = index_type(1) << 24;
+        constexpr index_type chunk = index_type(1) << 26;
         const index_type cap = num_in < chunk ? num_in : chunk;
         if (!scratch_.reserve(cap)) {   // no memory for the combining buffers: insert each occurrence
             insert_each<view_type, StatusHandler, false>(keys_in, nullptr, num_in, view(), stream, probing_length,
@@ -447,12 +519,20 @@ public:
         for (index_type first = 0; first < num_in; first += cap) {
             const index_type n = num_in - first < cap ? num_in - first : cap;
             combine(keys_in + first, n, buf, stream);
-            index_type entries = 0;
-            (void)hipMemcpyAsync(&entries, buf.num, sizeof(index_type), hipMemcpyDeviceToHost, stream);
-            (void)hipStreamSynchronize(stream);
-            insert_each<add_view_type, status_handlers::ReturnStatus, true>(
-                buf.keys, buf.counts, entries, add_view(), stream, probing_length, buf.status);
-            finish_combined<StatusHandler>(buf, first, n, entries, stream, status_out);
+            if constexpr (detail::fixed_step_scheme<ProbingScheme>::value) {
+                // the insert reads the entry count on the device: no host round trip
+                detail::launch_insert_refill<add_view_type, status_handlers::ReturnStatus,
+                                             detail::refill_insert_keys<unsigned(cg_size())>(), true>(
+                    buf.keys, buf.counts, n, add_view(), probing_length, buf.status, stream, buf.num);
+                finish_combined<StatusHandler>(buf, first, n, n, stream, status_out);
+            } else {
+                index_type entries = 0;
+                (void)hipMemcpyAsync(&entries, buf.num, sizeof(index_type), hipMemcpyDeviceToHost, stream);
+                (void)hipStreamSynchronize(stream);
+                insert_each<add_view_type, status_handlers::ReturnStatus, true>(
+                    buf.keys, buf.counts, entries, add_view(), stream, probing_length, buf.status);
+                finish_combined<StatusHandler>(buf, first, n, entries, stream, status_out);
+            }
         }
     }
 
@@ -466,7 +546,10 @@ public:
                                     hipStream_t stream = 0, index_type probing_length = defaults::probing_length(),
                                     typename StatusHandler::base_type* status_out = nullptr) noexcept {
         static_assert(detail::fixed_step_scheme<ProbingScheme>::value, "refill kernels need a fixed-step scheme");
-        if (!base_table_.is_initialized() || num_in == 0 || num_in > buf.capacity) return;
+        // entries index below 2^31 (the top bit of an occurrence's entry marks a hot-map entry)
+        if (!base_table_.is_initialized() || num_in == 0 || num_in > buf.capacity ||
+            num_in >= (index_type(1) << 31))
+            return;
         combine(keys_in, num_in, buf, stream);
         detail::launch_insert_refill<add_view_type, status_handlers::ReturnStatus,
                                      detail::refill_insert_keys<unsigned(cg_size())>(), true>(
@@ -479,11 +562,19 @@ private:
     // in *buf.num, cleared first by a fill kernel)
     WC_H void combine(const key_type* keys_in, index_type n, const CombineBuffers& buf, hipStream_t stream) noexcept {
         storage::kernels::fill<index_type><<<1, 64, 0, stream>>>(buf.num, 1, index_type(0));
+        constexpr auto k = kernels::count_combine<key_type, value_type, EmptyKey, TombstoneKey>;
+        // one round of resident blocks (a second round would run as a tail),
+        // each combining many tiles in its hot map
+        static const index_type per_cu = [] {
+            int b = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, 256, 0) != hipSuccess || b < 1) b = 1;
+            return index_type(b);
+        }();
         const index_type tiles = SDIV(n, index_type(detail::kCombineTile));
-        const index_type limit = index_type(detail::num_compute_units()) * 4;
-        kernels::count_combine<key_type, value_type, EmptyKey, TombstoneKey>
-            <<<unsigned(tiles < limit ? tiles : limit), 256, 0, stream>>>(keys_in, n, buf.keys, buf.counts, buf.rep,
-                                                                          buf.pos, buf.num);
+        index_type limit = index_type(detail::num_compute_units()) * per_cu;
+        if (limit > index_type(detail::kCombineBlocks)) limit = detail::kCombineBlocks;
+        k<<<unsigned(tiles < limit ? tiles : limit), 256, 0, stream>>>(keys_in, n, buf.keys, buf.counts, buf.rep,
+                                                                      buf.pos, buf.num, buf.hot);
     }
 
     // table-status join of duplicates and the per-occurrence statuses of
@@ -498,7 +589,7 @@ private:
         if constexpr (!std::is_same<StatusHandler, status_handlers::ReturnNothing>::value)
             if (status_out != nullptr)
                 kernels::count_expand<StatusHandler><<<detail::grid_for(n, 1, 8), block_size(), 0, stream>>>(
-                    buf.pos, buf.rep, buf.status, first, n, status_out);
+                    buf.pos, buf.hot, buf.rep, buf.status, first, n, status_out);
     }
 
 public:
