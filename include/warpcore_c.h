@@ -741,13 +741,16 @@ WC_API void wc_set_lane_insert_table_bytes(uint64_t bytes);
 WC_API uint64_t wc_get_lane_insert_table_bytes(void);
 
 /* Bulk kernels of tables probed by cooperative groups (no reference
- * counterpart), a bit mask: bit 3 = refill inserts, bit 2 = refill lookups,
- * bit 0 = lockstep inserts, bit 1 = lockstep lookups and erases (a refill
- * bit wins over the lockstep one; no bit = the streaming per-key state
- * machine). Default 12 (refill inserts and lookups). MultiValueHashTable
- * bulk calls use the refill kernels under bits 3 (inserts) and 2 (count and
- * gather passes), else one key per group. All choices give identical
- * results. */
+ * counterpart), a bit mask: bit 7 = lane-owner inserts, bit 6 = lane-owner
+ * lookups and erases (single-value tables with windows of 2..16 slots;
+ * these win over the rest), bit 8 = lane-owner kernels read packed 8-byte
+ * slots two per lane, bit 3 = refill inserts, bit 2 = refill lookups,
+ * bit 4 = pipelined refill lookups, bit 0 = lockstep inserts, bit 1 =
+ * lockstep lookups and erases (a refill bit wins over the lockstep one; no
+ * bit = the streaming per-key state machine). Default 476 (bits 2, 3, 4, 6,
+ * 7, 8). MultiValueHashTable bulk calls use the refill kernels under bits 3
+ * (inserts) and 2 (count and gather passes), else one key per group. All
+ * choices give identical results. */
 WC_API void wc_set_group_kernel(int kernel);
 WC_API int wc_get_group_kernel(void);
 

@@ -31,7 +31,7 @@ def oracle():
     return o
 
 
-@pytest.fixture(params=["lanes", "groups", "lockstep", "refill", "pipe", "owner"])
+@pytest.fixture(params=["lanes", "groups", "lockstep", "refill", "pipe", "owner", "pairs"])
 def shape(request):
     """Runs a GPU test once per bulk-kernel shape."""
     from datagen import kernel_shape

@@ -85,13 +85,14 @@ class kernel_shape:
     lookups: each slot's result consumed as soon as its own request
     returns) or
     'owner' (lane-owner kernels: every lane owns one key, groups read their
-    owners' windows in turn), by
+    owners' windows in turn) or 'pairs' (lane-owner kernels reading packed
+    8-byte slots two per lane, the default), by
     moving the table-size limit of
     the lane-per-key path and choosing the group kernels."""
 
     def __init__(self, shape: str):
         self.limit = (1 << 62) if shape == "lanes" else 0
-        self.group_kernel = {"groups": 0, "refill": 12, "pipe": 28, "owner": 220}.get(shape, 3)
+        self.group_kernel = {"groups": 0, "refill": 12, "pipe": 28, "owner": 220, "pairs": 476}.get(shape, 3)
 
     def __enter__(self):
         from warpcore_amd import _capi

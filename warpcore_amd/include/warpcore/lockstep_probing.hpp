@@ -553,7 +553,7 @@ WC_H unsigned grid_for_lockstep(index_t items) noexcept {
 // factor 0.9 is many windows; refill slots do not wait; a pipelined refill
 // insert measured 11.3 and 11.7 and was dropped).
 inline std::atomic<int>& group_kernel_setting() noexcept {
-    static std::atomic<int> k{220};
+    static std::atomic<int> k{476};
     return k;
 }
 WC_H int group_kernel() noexcept { return group_kernel_setting().load(std::memory_order_relaxed); }
@@ -568,6 +568,9 @@ WC_H bool pipelined_lookups() noexcept { return (group_kernel() & 16) != 0; }
 // single-value tables; before the refill bits)
 WC_H bool owner_lookups() noexcept { return (group_kernel() & 64) != 0; }
 WC_H bool owner_inserts() noexcept { return (group_kernel() & 128) != 0; }
+// bit 8: lane-owner kernels read packed 8-byte slots two per lane (a window
+// of W slots by W/2 lanes, one 16-byte load each)
+WC_H bool owner_pairs() noexcept { return (group_kernel() & 256) != 0; }
 
 template<class Core, class StatusHandler, int U>
 WC_H void launch_erase_lockstep(const typename Core::key_type* keys_in, index_t num_in, const Core& core,

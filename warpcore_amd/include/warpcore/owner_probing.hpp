@@ -444,7 +444,7 @@ namespace kernels {
 
 // Bulk inserts / lookups by lane owners: wavefront v of the grid takes keys
 // [v * per, (v + 1) * per).
-template<class Core, class StatusHandler, class W, bool HasValues>
+template<class Core, class StatusHandler, class W, bool HasValues, unsigned SPL = 1>
 __global__ __launch_bounds__(256) void insert_owner(const typename Core::key_type* __restrict__ keys_in,
                                                     const typename Core::value_type* __restrict__ values_in,
                                                     index_t num_in, Core core, index_t probing_length,
@@ -460,12 +460,12 @@ __global__ __launch_bounds__(256) void insert_owner(const typename Core::key_typ
     const index_t end = first + per < num_in ? first + per : num_in;
     const index_t budget = probing_schemes::detail::window_budget(probing_length, Core::cg_size(),
                                                                   core.geometry().windows);
-    detail::insert_owner_range<Core, StatusHandler, W, HasValues>(keys_in, values_in, first, end, core, budget,
+    detail::insert_owner_range<Core, StatusHandler, W, HasValues, SPL>(keys_in, values_in, first, end, core, budget,
                                                                   status_out, lds + (threadIdx.x & ~63u), join);
     join.flush(core.status_word());
 }
 
-template<class Core, class StatusHandler, class W>
+template<class Core, class StatusHandler, class W, unsigned SPL = 1>
 __global__ __launch_bounds__(256) void retrieve_owner(const typename Core::key_type* __restrict__ keys_in,
                                                       index_t num_in, typename Core::out_type* __restrict__ values_out,
                                                       Core core, index_t probing_length,
@@ -479,12 +479,12 @@ __global__ __launch_bounds__(256) void retrieve_owner(const typename Core::key_t
     const index_t end = first + per < num_in ? first + per : num_in;
     const index_t budget = probing_schemes::detail::window_budget(probing_length, Core::cg_size(),
                                                                   core.geometry().windows);
-    detail::retrieve_owner_range<Core, StatusHandler, W>(keys_in, first, end, values_out, core, budget, status_out,
+    detail::retrieve_owner_range<Core, StatusHandler, W, SPL>(keys_in, first, end, values_out, core, budget, status_out,
                                                          lds + (threadIdx.x & ~63u), join);
     join.flush(core.status_word());
 }
 
-template<class Core, class StatusHandler, class W>
+template<class Core, class StatusHandler, class W, unsigned SPL = 1>
 __global__ __launch_bounds__(256) void erase_owner(const typename Core::key_type* __restrict__ keys_in,
                                                    index_t num_in, Core core, index_t probing_length,
                                                    typename StatusHandler::base_type* __restrict__ status_out) {
@@ -497,7 +497,7 @@ __global__ __launch_bounds__(256) void erase_owner(const typename Core::key_type
     const index_t end = first + per < num_in ? first + per : num_in;
     const index_t budget = probing_schemes::detail::window_budget(probing_length, Core::cg_size(),
                                                                   core.geometry().windows);
-    detail::retrieve_owner_range<Core, StatusHandler, W, 1, true>(keys_in, first, end, nullptr, core, budget,
+    detail::retrieve_owner_range<Core, StatusHandler, W, SPL, true>(keys_in, first, end, nullptr, core, budget,
                                                                   status_out, lds + (threadIdx.x & ~63u), join);
     join.flush(core.status_word());
 }

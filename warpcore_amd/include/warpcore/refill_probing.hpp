@@ -735,48 +735,106 @@ WC_H void launch_insert_refill_as(const typename Core::key_type* keys_in, const 
     }
 }
 
+// Slots per lane of the lane-owner kernels for a core: 2 (a window read by
+// half as many lanes, one 16-byte load each) for packed 8-byte slots in
+// windows of 4..16 slots when owner_pairs() is on, else 1.
+template<class C, class = void>
+struct has_packed_slots : std::false_type {};
+template<class C>
+struct has_packed_slots<C, std::void_t<decltype(C::packed)>> : std::integral_constant<bool, C::packed> {};
+
+template<class Core>
+constexpr bool owner_pairs_capable() noexcept {
+    return has_packed_slots<Core>::value && !is_multi_value_core<Core>::value && Core::cg_size() >= 4 &&
+           Core::cg_size() <= 16;
+}
+
+template<class Core, class StatusHandler, bool HasValues, unsigned SPL>
+WC_H void launch_insert_owner_as(const typename Core::key_type* keys_in, const typename Core::value_type* values_in,
+                                 index_t num_in, const Core& core, index_t probing_length,
+                                 typename StatusHandler::base_type* status_out, hipStream_t stream,
+                                 const index_t* num_dev) noexcept {
+    if (core.geometry().windows < (index_t(1) << 31)) {
+        constexpr auto k = kernels::insert_owner<Core, StatusHandler, std::uint32_t, HasValues, SPL>;
+        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length,
+                                                              status_out, num_dev);
+    } else {
+        constexpr auto k = kernels::insert_owner<Core, StatusHandler, index_t, HasValues, SPL>;
+        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length,
+                                                              status_out, num_dev);
+    }
+}
+
 template<class Core, class StatusHandler, bool HasValues>
 WC_H void launch_insert_owner(const typename Core::key_type* keys_in, const typename Core::value_type* values_in,
                               index_t num_in, const Core& core, index_t probing_length,
                               typename StatusHandler::base_type* status_out, hipStream_t stream,
                               const index_t* num_dev) noexcept {
-    if (core.geometry().windows < (index_t(1) << 31)) {
-        constexpr auto k = kernels::insert_owner<Core, StatusHandler, std::uint32_t, HasValues>;
-        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length,
-                                                              status_out, num_dev);
-    } else {
-        constexpr auto k = kernels::insert_owner<Core, StatusHandler, index_t, HasValues>;
-        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, values_in, num_in, core, probing_length,
-                                                              status_out, num_dev);
+    if constexpr (owner_pairs_capable<Core>()) {
+        if (owner_pairs()) {
+            launch_insert_owner_as<Core, StatusHandler, HasValues, 2>(keys_in, values_in, num_in, core,
+                                                                      probing_length, status_out, stream, num_dev);
+            return;
+        }
     }
+    launch_insert_owner_as<Core, StatusHandler, HasValues, 1>(keys_in, values_in, num_in, core, probing_length,
+                                                              status_out, stream, num_dev);
+}
+
+template<class Core, class StatusHandler, unsigned SPL, bool Erase>
+WC_H void launch_lookup_owner_as(const typename Core::key_type* keys_in, index_t num_in,
+                                 typename Core::out_type* values_out, const Core& core, index_t probing_length,
+                                 typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
+    if (core.geometry().windows < (index_t(1) << 31)) {
+        if constexpr (Erase) {
+            constexpr auto k = kernels::erase_owner<Core, StatusHandler, std::uint32_t, SPL>;
+            k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, num_in, core, probing_length, status_out);
+        } else {
+            constexpr auto k = kernels::retrieve_owner<Core, StatusHandler, std::uint32_t, SPL>;
+            k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length,
+                                                                  status_out);
+        }
+    } else {
+        if constexpr (Erase) {
+            constexpr auto k = kernels::erase_owner<Core, StatusHandler, index_t, SPL>;
+            k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, num_in, core, probing_length, status_out);
+        } else {
+            constexpr auto k = kernels::retrieve_owner<Core, StatusHandler, index_t, SPL>;
+            k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length,
+                                                                  status_out);
+        }
+    }
+}
+
+template<class Core, class StatusHandler, bool Erase = false>
+WC_H void launch_lookup_owner(const typename Core::key_type* keys_in, index_t num_in,
+                              typename Core::out_type* values_out, const Core& core, index_t probing_length,
+                              typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
+    if constexpr (owner_pairs_capable<Core>()) {
+        if (owner_pairs()) {
+            launch_lookup_owner_as<Core, StatusHandler, 2, Erase>(keys_in, num_in, values_out, core, probing_length,
+                                                                  status_out, stream);
+            return;
+        }
+    }
+    launch_lookup_owner_as<Core, StatusHandler, 1, Erase>(keys_in, num_in, values_out, core, probing_length,
+                                                          status_out, stream);
 }
 
 template<class Core, class StatusHandler>
 WC_H void launch_retrieve_owner(const typename Core::key_type* keys_in, index_t num_in,
                                 typename Core::out_type* values_out, const Core& core, index_t probing_length,
                                 typename StatusHandler::base_type* status_out, hipStream_t stream) noexcept {
-    if (core.geometry().windows < (index_t(1) << 31)) {
-        constexpr auto k = kernels::retrieve_owner<Core, StatusHandler, std::uint32_t>;
-        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length,
-                                                              status_out);
-    } else {
-        constexpr auto k = kernels::retrieve_owner<Core, StatusHandler, index_t>;
-        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, num_in, values_out, core, probing_length,
-                                                              status_out);
-    }
+    launch_lookup_owner<Core, StatusHandler, false>(keys_in, num_in, values_out, core, probing_length, status_out,
+                                                    stream);
 }
 
 template<class Core, class StatusHandler>
 WC_H void launch_erase_owner(const typename Core::key_type* keys_in, index_t num_in, const Core& core,
                              index_t probing_length, typename StatusHandler::base_type* status_out,
                              hipStream_t stream) noexcept {
-    if (core.geometry().windows < (index_t(1) << 31)) {
-        constexpr auto k = kernels::erase_owner<Core, StatusHandler, std::uint32_t>;
-        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, num_in, core, probing_length, status_out);
-    } else {
-        constexpr auto k = kernels::erase_owner<Core, StatusHandler, index_t>;
-        k<<<owner_grid<k>(num_in), block_size(), 0, stream>>>(keys_in, num_in, core, probing_length, status_out);
-    }
+    launch_lookup_owner<Core, StatusHandler, true>(keys_in, num_in, nullptr, core, probing_length, status_out,
+                                                   stream);
 }
 
 // Lane-owner kernels apply to single-value cores whose windows are 2..16
