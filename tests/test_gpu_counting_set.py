@@ -141,6 +141,57 @@ def test_hashset_parity(cuda, shape, oracle, kb, cg):
     np.testing.assert_array_equal(np.sort(from_dev(ko, dtype(kb))[:m]), np.sort(keys[n // 2:]))
 
 
+@pytest.mark.parametrize("kb,cg", [(32, 16), (64, 8), (32, 4)])
+def test_hashset_high_load(cuda, shape, oracle, kb, cg):
+    """Membership at high load, where most absent keys walk several full
+    windows: flags and statuses equal the oracle's at load 0.97, after
+    erasing a third and re-inserting some of it (tombstones are never
+    reused, so the re-inserts take EMPTY slots further along), and once the
+    last EMPTY slot is taken (every absent key then walks the whole cycle and
+    reports probing_length_exceeded)."""
+    from warpcore_amd.tables import HashSet
+    print(f"[hashset high load] kb={kb} cg={cg} shape={shape}", flush=True)
+    cap = oracle.get_valid_capacity(1 << 16, cg)
+    n = int(cap * 0.97)
+    keys = unique_keys(n, kb, seed=71)
+    absent = unique_keys(n, kb, seed=72, exclude=keys)
+    t = HashSet(1 << 16, kb, cg_size=cg)
+    o = oracle.HashSet(1 << 16, kb, cg=cg)
+    assert t.capacity() == cap
+
+    def check(q):
+        f = dev_empty(len(q), 8, 7)
+        qs = dev_empty(len(q), 32)
+        t.contains(to_dev(q), f, status_out=qs)
+        ef, es = o.contains(q)
+        np.testing.assert_array_equal(from_dev(qs, np.uint32), es)
+        gf = from_dev(f, np.uint8)
+        np.testing.assert_array_equal(gf[es == S_NONE], ef[es == S_NONE])
+        return es
+
+    st = dev_empty(n, 32)
+    t.insert(to_dev(keys), status_out=st)
+    np.testing.assert_array_equal(from_dev(st, np.uint32), o.insert(keys))
+    q = np.concatenate([keys[::2], absent])
+    check(q)
+    er = keys[::3]
+    est = dev_empty(len(er), 32)
+    t.erase(to_dev(er), status_out=est)
+    np.testing.assert_array_equal(from_dev(est, np.uint32), o.erase(er))
+    back = er[: (cap - n) // 2]
+    st = dev_empty(len(back), 32)
+    t.insert(to_dev(back), status_out=st)
+    np.testing.assert_array_equal(from_dev(st, np.uint32), o.insert(back))
+    check(q)
+    assert t.peek_status() == o.status()
+    rest = unique_keys(cap - n - len(back), kb, seed=73, exclude=np.concatenate([keys, absent]))
+    st = dev_empty(len(rest), 32)
+    t.insert(to_dev(rest), status_out=st)
+    np.testing.assert_array_equal(from_dev(st, np.uint32), o.insert(rest))
+    assert t.size() == o.size()
+    assert np.all(check(absent[:64]) == S_PLE)
+
+
 @pytest.mark.parametrize("bits", [32, 64])
 def test_unique_generator(cuda, oracle, bits):
     n = 1 << 20
