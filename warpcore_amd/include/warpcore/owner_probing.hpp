@@ -96,9 +96,9 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
     constexpr unsigned WIN = unsigned(Core::cg_size());   // slots per window
     constexpr unsigned CG = WIN / SPL;                     // lanes per group
     static_assert(CG >= 2 && CG <= 64 && (CG & (CG - 1)) == 0, "groups of 2..64 lanes");
-    static_assert(!is_multi_value_core<Core>::value, "single-value cores only");
-    static_assert(SPL == 1 || (SPL == 2 && sizeof(Probe) == 8 && !std::is_same<Probe, K>::value),
-                  "two slots per lane: packed 8-byte slots");
+    constexpr bool multi = is_multi_value_core<Core>::value;
+    static_assert(SPL == 1 || (SPL == 2 && !multi && sizeof(Probe) == 8 && !std::is_same<Probe, K>::value),
+                  "two slots per lane: packed 8-byte slots of single-value cores");
     constexpr bool keep_seen = !std::is_same<Probe, K>::value;
     constexpr unsigned kFree = 0, kLoad = 1, kClaim = 2;
     const unsigned lane = lane_id();
@@ -124,6 +124,8 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
     unsigned phase = kFree;
     Bits emp = 0;
     Probe seen = Core::empty_probe();
+    index_t count = 0;      // multi-value: values of the key met so far
+    index_t new_keys = 0;   // multi-value: keys this lane stored a first value of
     // the lane's next key, loaded one key ahead
     index_t cursor = first;
     K nk = K{};
@@ -139,8 +141,11 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
     auto advance = [&]() {   // to the key's next window, or out of windows
         mark_overflow(core, w, step);
         if (rnd + 1 >= budget) {
-            join(Status::probing_length_exceeded());
-            StatusHandler::handle(Status::probing_length_exceeded(), status_out, pos);
+            Status st = Status::probing_length_exceeded();
+            if constexpr (multi)
+                if (count > 0) st = st + Status::duplicate_key();
+            join(st);
+            StatusHandler::handle(st, status_out, pos);
             phase = kFree;
         } else {
             const W nw = w + step;
@@ -163,6 +168,7 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
                 w = W(c.w);
                 step = W(c.step);
                 rnd = 0;
+                count = 0;
                 phase = kLoad;
             } else {
                 join(Status::invalid_key());
@@ -209,18 +215,46 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
             if constexpr (SPL == 1) probe[j][0] = Core::load_insert_probe(slots + index_t(at) * WIN + rank);
             else owner_load_pair<SPL>(slots + index_t(at) * WIN + rank * SPL, probe[j]);
         }
-        if (ph == kClaim) {
-            const int code = core.cas_outcome(old, seen, key);
-            if (code == 1 || code == 2) {   // claimed, or the key was already there
-                Status st = code == 1 ? Status::none() : Status::duplicate_key();
-                core.finish_insert(index_t(w) * WIN + lead, val, code == 1 && Core::publishes_value, st, join);
-                if (code == 2) join(Status::duplicate_key());
-                StatusHandler::handle(st, status_out, pos);
-                phase = kFree;
-            } else if (code == 0) {   // another key took the slot: the window's next EMPTY slot
-                emp &= Bits(emp - 1);
-                if (emp == 0) advance();
-            }                         // code 3: the same slot again, with its refreshed word
+        if constexpr (multi) {
+            if (ph == kClaim) {
+                // multi_value_hash_table.cuh:200-289 per key: a claim stores one
+                // more value; a slot that took this key meanwhile counts as one
+                // of its values
+                const int code = core.cas_outcome(old, seen, key);
+                if (code == 1) {
+                    const Status st = count > 0 ? Status::duplicate_key() : Status::none();
+                    Status fin = st;
+                    core.finish_insert(index_t(w) * WIN + lead, val, Core::publishes_value, fin, join);
+                    if (count == 0) ++new_keys;
+                    StatusHandler::handle(st, status_out, pos);
+                    phase = kFree;
+                } else if (code == 0 || code == 2) {
+                    if (code == 2) count += 1;
+                    if (count >= index_t(core.max_values)) {
+                        const Status full = Status::duplicate_key() + Status::max_values_for_key_reached();
+                        join(full);
+                        StatusHandler::handle(full, status_out, pos);
+                        phase = kFree;
+                    } else {
+                        emp &= Bits(emp - 1);
+                        if (emp == 0) advance();
+                    }
+                }
+            }
+        } else {
+            if (ph == kClaim) {
+                const int code = core.cas_outcome(old, seen, key);
+                if (code == 1 || code == 2) {   // claimed, or the key was already there
+                    Status st = code == 1 ? Status::none() : Status::duplicate_key();
+                    core.finish_insert(index_t(w) * WIN + lead, val, code == 1 && Core::publishes_value, st, join);
+                    if (code == 2) join(Status::duplicate_key());
+                    StatusHandler::handle(st, status_out, pos);
+                    phase = kFree;
+                } else if (code == 0) {   // another key took the slot: the window's next EMPTY slot
+                    emp &= Bits(emp - 1);
+                    if (emp == 0) advance();
+                }                         // code 3: the same slot again, with its refreshed word
+            }
         }
         __builtin_amdgcn_wave_barrier();
 
@@ -242,23 +276,42 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
             }
         }
 
-        if (ph == kLoad) {
-            if (hit != 0) {   // duplicate: the value is overwritten in place
-                Status st = Status::duplicate_key();
-                core.finish_insert(index_t(w) * WIN + owner_slot_of_bit<CG, SPL>(unsigned(ffs64(std::uint64_t(hit)) - 1)),
-                                   val, false, st, join);
-                join(Status::duplicate_key());
-                StatusHandler::handle(st, status_out, pos);
-                phase = kFree;
-            } else if (empty != 0) {
-                emp = empty;
-                if constexpr (keep_seen) seen = Core::empty_probe();
-                phase = kClaim;
-            } else {
-                advance();
+        if constexpr (multi) {
+            if (ph == kLoad) {
+                count += index_t(__builtin_popcountll(std::uint64_t(hit)));
+                if (count >= index_t(core.max_values)) {
+                    const Status full = Status::duplicate_key() + Status::max_values_for_key_reached();
+                    join(full);
+                    StatusHandler::handle(full, status_out, pos);
+                    phase = kFree;
+                } else if (empty != 0) {
+                    emp = empty;
+                    if constexpr (keep_seen) seen = Core::empty_probe();
+                    phase = kClaim;
+                } else {
+                    advance();
+                }
+            }
+        } else {
+            if (ph == kLoad) {
+                if (hit != 0) {   // duplicate: the value is overwritten in place
+                    Status st = Status::duplicate_key();
+                    core.finish_insert(index_t(w) * WIN + owner_slot_of_bit<CG, SPL>(unsigned(ffs64(std::uint64_t(hit)) - 1)),
+                                       val, false, st, join);
+                    join(Status::duplicate_key());
+                    StatusHandler::handle(st, status_out, pos);
+                    phase = kFree;
+                } else if (empty != 0) {
+                    emp = empty;
+                    if constexpr (keep_seen) seen = Core::empty_probe();
+                    phase = kClaim;
+                } else {
+                    advance();
+                }
             }
         }
     }
+    if constexpr (multi) core.add_keys(new_keys);
 }
 
 // Lookups of keys_in[first .. end) by the calling wavefront. A hit is

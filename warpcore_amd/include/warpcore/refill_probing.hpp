@@ -844,13 +844,18 @@ template<class Core>
 constexpr bool owner_capable() noexcept {
     return !is_multi_value_core<Core>::value && Core::cg_size() >= 2 && Core::cg_size() <= 16;
 }
+// Inserts also for multi-value cores (a claim stores one more value).
+template<class Core>
+constexpr bool owner_insert_capable() noexcept {
+    return Core::cg_size() >= 2 && Core::cg_size() <= 16;
+}
 
 template<class Core, class StatusHandler, int U, bool HasValues>
 WC_H void launch_insert_refill(const typename Core::key_type* keys_in, const typename Core::value_type* values_in,
                                index_t num_in, const Core& core, index_t probing_length,
                                typename StatusHandler::base_type* status_out, hipStream_t stream,
                                const index_t* num_dev = nullptr) noexcept {
-    if constexpr (owner_capable<Core>()) {
+    if constexpr (owner_insert_capable<Core>()) {
         if (owner_inserts()) {
             launch_insert_owner<Core, StatusHandler, HasValues>(keys_in, values_in, num_in, core, probing_length,
                                                                 status_out, stream, num_dev);
