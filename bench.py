@@ -536,6 +536,44 @@ def run_bloom(g: Gpu, log2_bits: int = 30, k: int = 8, log2n: int = 27, reps: in
     return res
 
 
+def run_bucketlist(g: Gpu, log2n: int = 27, mult: int = 8, reps: int = 3):
+    """BucketListHashTable<u32,u32> of the reference's bucket-list benchmark
+    shape: 2^27 values on 2^24 keys (each key 8 times, shuffled), key
+    capacity 2^24 / 0.9, value capacity 2^27 / 0.5, buckets growing by 1.1
+    from 1 value, cg 8; insert, then the two-pass retrieve of every key."""
+    torch = g.torch
+    from warpcore_amd.tables import BucketListHashTable
+    n = 1 << log2n
+    d = n // mult
+    ukeys = g.empty(d, 32)
+    with torch.cuda.stream(g.stream):
+        g.check(g.lib.wc_gen_unique_u32(g.p(ukeys), d, 0x6000, g.sp), "gen keys")
+    g.stream.synchronize()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0xC6)
+    pk = ukeys.repeat(mult)[torch.randperm(n, device="cuda", generator=gen)].contiguous()
+    pv = g.empty(n, 32)
+    with torch.cuda.stream(g.stream):
+        g.check(g.lib.wc_gen_uniform_u32(g.p(pv), n, 0xC7, g.sp), "gen values")
+    torch.cuda.synchronize()
+    t = BucketListHashTable(math.ceil(d / 0.9), math.ceil(n / 0.5), 32, 32, grow_factor=1.1, min_bucket_size=1,
+                            cg_size=8)
+    g.stream.synchronize()
+    ins, _ = _timed(g, lambda: t.insert(pk, pv, stream=g.stream), reps, before=lambda: t.init(stream=g.stream))
+    beg = torch.empty(d, dtype=torch.int64, device="cuda")
+    end = torch.empty(d, dtype=torch.int64, device="cuda")
+    vout = g.empty(n, 32)
+    got = []
+    ret, _ = _timed(g, lambda: got.append(t.retrieve(ukeys, beg, end, vout, stream=g.stream)), reps)
+    if got[-1] != n:
+        raise RuntimeError(f"bucket-list result check failed: {got[-1]} values of {n}")
+    res = {"keys": d, "values": n, "insert_gops": round(n / ins / 1e6, 3), "insert_ms": round(ins, 4),
+           "retrieve_values_gops": round(n / ret / 1e6, 3), "retrieve_ms": round(ret, 4)}
+    del t, ukeys, pk, pv, beg, end, vout
+    torch.cuda.empty_cache()
+    return res
+
+
 def run_mvht(g: Gpu, log2n: int = 26, mean: int = 8, reps: int = 3):
     """MultiValueHashTable<u32,u32>: 2^26 pairs, per-key multiplicity geometric
     with mean 8 (about 2^23 keys), load factor 0.9 (75,505,624 slots); insert,
@@ -700,7 +738,9 @@ def main():
             extras["hashset_u32_2p27"] = run_hashset(g)
             extras["bloom_2p30bits_k8"] = run_bloom(g)
             extras["mvht_u32u32_2p26"] = run_mvht(g)
-            for k in ("svht_u32u32_2p20", "hashset_u32_2p27", "bloom_2p30bits_k8", "mvht_u32u32_2p26"):
+            extras["bucketlist_u32u32_2p27"] = run_bucketlist(g)
+            for k in ("svht_u32u32_2p20", "hashset_u32_2p27", "bloom_2p30bits_k8", "mvht_u32u32_2p26",
+                      "bucketlist_u32u32_2p27"):
                 log(f"[bench] {k}: {extras[k]}")
         if not args.no_serving:
             log("[bench] serving ...")

@@ -198,7 +198,10 @@ class _Handle:
     def __del__(self):
         h = getattr(self, "h", None)
         if h:
-            self._f("destroy")(h)
+            try:
+                self._f("destroy")(h)
+            except TypeError:   # interpreter shutdown: the module's globals are gone
+                pass
             self.h = None
 
 

@@ -135,6 +135,77 @@ def test_bucketlist_out_of_memory(cuda, oracle):
     np.testing.assert_array_equal(vals[:total], ov)
 
 
+def _arenas(value_cap):
+    """Arenas of a value pool: the largest power of two up to 256 leaving
+    every arena at least 2^16 cells."""
+    a = 1
+    while a < 256 and value_cap // (2 * a) >= 1 << 16:
+        a *= 2
+    return a
+
+
+def test_bucketlist_arenas_match_oracle(cuda, oracle):
+    """A pool of 64 arenas: bucket sizes still depend only on each key's
+    count, so pool usage, offsets and value multisets equal the oracle's."""
+    from warpcore_amd.tables import BucketListHashTable
+    keys, mult, k, v = _pairs(1 << 17, 32, 8.0, seed=41)
+    key_cap, value_cap = int(len(keys) / 0.8), 4 * len(k)
+    assert _arenas(value_cap) == 64
+    t = BucketListHashTable(key_cap, value_cap, 32, 32, grow_factor=1.1, min_bucket_size=1, cg_size=8)
+    empty = BucketListHashTable(key_cap, 1000, 32, 32, grow_factor=1.1, min_bucket_size=1, cg_size=8)
+    # the counters of 64 arenas take one 128-byte line each; one arena, 8 bytes
+    assert t.bytes_total() - empty.bytes_total() == (value_cap - 1000) * 8 + 64 * 128 - 8
+    o = oracle.BucketList(key_cap, value_cap, 32, 32, cg=8, grow=1.1, min_bucket=1)
+    st = dev_empty(len(k), 32, 0x77)
+    t.insert(to_dev(k), to_dev(v), status_out=st)
+    s_gpu = from_dev(st, np.uint32)
+    s_ora = o.insert(k, v)
+    assert np.sum(s_gpu == S_NONE) == np.sum(s_ora == S_NONE) == len(keys)
+    assert set(np.unique(s_gpu)) <= {S_NONE, S_DUP}
+    assert t.num_values() == len(k)
+    assert t.value_cells_used() == o.next_free()
+    total, b, e, vals, qst = _retrieve(t, keys, len(k), 32)
+    ob, oe, ov, ost = o.retrieve(keys)
+    assert total == len(k)
+    np.testing.assert_array_equal(b, ob)
+    np.testing.assert_array_equal(e, oe)
+    order = np.argsort(k, kind="stable")
+    ks, vs = k[order], v[order]
+    starts = np.searchsorted(ks, keys)
+    for i in np.random.default_rng(3).choice(len(keys), 2000, replace=False):
+        want = np.sort(vs[starts[i]:starts[i] + mult[i]])
+        np.testing.assert_array_equal(np.sort(vals[int(b[i]):int(e[i])]), want)
+    np.testing.assert_array_equal(np.sort(vals[:total]), np.sort(v))
+
+
+def test_bucketlist_arenas_out_of_memory(cuda):
+    """An allocation fails only when no arena holds it: afterwards every
+    arena counts as used to its end, the pairs not reported out_of_memory
+    are all stored, and each key returns distinct values of its own."""
+    from warpcore_amd.tables import BucketListHashTable
+    keys, mult, k, v = _pairs(1 << 16, 32, 8.0, seed=43)
+    value_cap = len(k) // 2
+    assert _arenas(value_cap) >= 4
+    t = BucketListHashTable(int(len(keys) / 0.8), value_cap, 32, 32, grow_factor=1.5, min_bucket_size=2,
+                            max_bucket_size=32, cg_size=8)
+    st = dev_empty(len(k), 32)
+    t.insert(to_dev(k), to_dev(v), status_out=st)
+    s = from_dev(st, np.uint32)
+    oom = (s & S_OOM) != 0
+    assert oom.any() and (~oom).any()
+    assert set(np.unique(s & ~np.uint32(S_OOM))) <= {S_NONE, S_DUP}
+    assert t.num_values() == np.sum(~oom)
+    assert t.value_cells_used() == value_cap
+    total, b, e, vals, _ = _retrieve(t, keys, len(k), 32)
+    assert total == np.sum(~oom)
+    stored = set(v[~oom].tolist())
+    assert set(vals[:total].tolist()) == stored
+    exp = _expected(k, v)
+    for i in np.random.default_rng(4).choice(len(keys), 1000, replace=False):
+        got = vals[int(b[i]):int(e[i])].tolist()
+        assert len(set(got)) == len(got) and set(got) <= set(exp[int(keys[i])])
+
+
 def test_bucketlist_max_values_per_key(cuda, oracle):
     from warpcore_amd.tables import BucketListHashTable
     keys, mult, k, v = _pairs(2000, 64, 6.0, seed=13)

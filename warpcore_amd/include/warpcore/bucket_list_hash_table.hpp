@@ -5,7 +5,7 @@
 // Keys live in a SingleValueHashTable<Key, handle>; a handle is one 64-bit
 // word {state: 2 bits, bucket index, value count, values in the tail
 // bucket} updated only by CAS/exchange. Values live in a pool of 8-byte
-// cells handed out by an atomicAdd on next_free. A key's first bucket holds
+// cells handed out by atomicAdds on allocation counters. A key's first bucket holds
 // min_bucket_size values and has no header; every later bucket is a header
 // cell {previous bucket, bucket size} followed by its values, each bucket
 // ceil(previous size * grow_factor) cells long, capped at max_bucket_size.
@@ -18,6 +18,15 @@
 // handle is always released; the re-read loop is also bounded and returns
 // unknown_error if that bound is ever reached. Otherwise the CAS bumps the
 // count and the tail, and the winner stores the value in its cell.
+//
+// Allocation: the reference bumps one counter for the whole pool. Device
+// atomics on one address are serialised at the memory side (12 ns each on
+// MI355X, measured), which bounded inserts at 2.8 G values/s; so pools of
+// at least 2^17 cells are cut into up to 256 arenas of at least 2^16 cells,
+// each with its own counter on its own 128-byte line. A wavefront allocates
+// from its home arena and moves on to the next ones when a request does not
+// fit; an allocation fails (out_of_memory) only when it fits in none. A
+// smaller pool is one arena and behaves exactly as the reference's.
 //
 // Values of a key come back tail bucket first (storage.cuh:1292-1345). A
 // handle left `full` by an out-of-memory append still yields the values it
@@ -33,7 +42,9 @@
 #include "status.hpp"
 #include "tile.hpp"
 
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace warpcore {
 namespace storage {
@@ -89,6 +100,32 @@ union alignas(alignof(Value) > 8 ? alignof(Value) : 8) Bucket {
     }
 };
 
+// atomic_add(counter, n) of every active lane of the wavefront with one
+// atomic: the lanes' amounts follow each other in lane order after the value
+// the counter had. The counter ends where one atomic per lane would leave it
+// and every lane's range lies below it exactly as one of those atomics would
+// have placed some lane's; only which lane gets which range differs. (A bump
+// allocator shared by the whole grid serialises its atomics at the memory
+// side; this divides them by the lanes allocating together.)
+WC_D index_t wave_atomic_add(index_t* counter, index_t n) noexcept {
+    const std::uint64_t active = __builtin_amdgcn_read_exec();
+    const unsigned lane = lane_id();
+    index_t before = 0, total = 0;
+    for (std::uint64_t m = active; m != 0; m &= m - 1) {
+        const unsigned l = unsigned(__builtin_ctzll(m));
+        const index_t v = (index_t(unsigned(__builtin_amdgcn_readlane(int(n >> 32), int(l)))) << 32) |
+                          index_t(unsigned(__builtin_amdgcn_readlane(int(n), int(l))));
+        if (l < lane) before += v;
+        total += v;
+    }
+    const unsigned leader = unsigned(__builtin_ctzll(active));
+    index_t base = 0;
+    if (lane == leader) base = warpcore::detail::atomic_add(counter, total);
+    base = (index_t(unsigned(__builtin_amdgcn_readlane(int(base >> 32), int(leader)))) << 32) |
+           index_t(unsigned(__builtin_amdgcn_readlane(int(base), int(leader))));
+    return base + before;
+}
+
 // Device half of the store, passed to kernels by value.
 template<class Value, index_t IB, index_t CB, index_t SB>
 struct BucketListStoreView {
@@ -99,30 +136,96 @@ struct BucketListStoreView {
     static constexpr std::uint32_t kMaxBlockedReads = 1u << 24;
 
     bucket_type* buckets;
-    index_t* next_free;
+    index_t* counters;    // arena a's cells handed out: counters[a * kCounterStride]
     index_t capacity;
     float grow;
     index_t min_size;
     index_t max_size;
+    index_t arenas;       // a power of two; every arena but the last holds arena_cells cells
+    index_t arena_cells;
+
+    static constexpr index_t kCounterStride = 16;   // 128 bytes between counters
+
+    WC_HD index_t arena_begin(index_t a) const noexcept { return a * arena_cells; }
+    WC_HD index_t arena_size(index_t a) const noexcept {
+        return a + 1 < arenas ? arena_cells : capacity - a * arena_cells;
+    }
+
+    // `n` cells for every active lane, from the wavefront's home arena or
+    // the ones after it; the first cell, or ~0 when no arena holds them.
+    // Active lanes take part together (a wave-uniform walk over the arenas).
+    WC_D index_t allocate(index_t n) const noexcept {
+        const index_t home = arenas == 1 ? 0
+            : index_t(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (arenas - 1);
+        for (index_t t = 0; t < arenas; ++t) {
+            const index_t a = (home + t) & (arenas - 1);
+            const index_t at = wave_atomic_add(counters + a * kCounterStride, n);
+            if (at + n <= arena_size(a)) return arena_begin(a) + at;
+        }
+        return ~index_t(0);
+    }
 
     WC_HD static constexpr index_t head() noexcept { return handle_type::max_bucket_index(); }
 
+    // A new bucket's link (info word) read by appends must be visible
+    // before the handle that points to it: an append that cannot size the
+    // tail from the count (size_by_count) reads the tail's link with a
+    // device-scope atomic load once it sees the handle. Both are written
+    // with device-scope atomics, which the memory side performs; the link's
+    // exchange returns its old word, and waiting for that word before the
+    // handle's exchange is issued orders the two at the memory side for
+    // every XCD. Values are plain stores, read only by later launches. (A
+    // device-scope fence here writes back and invalidates the L2 of every
+    // XCD: 2.3x the whole insert's time.)
+    WC_D static void wait_for(std::uint64_t returned) noexcept {
+        asm volatile("" ::"v"(unsigned(returned)) : "memory");
+    }
+
+    // Size of the bucket that holds value number c (c >= 1) of a key:
+    // buckets follow one sequence (min_size, then each the previous one
+    // grown by `grow`, at most max_size, computed as the allocation below
+    // does), so it is found from the count without reading the bucket's link
+    // when the sequence settles within kSizeSteps steps; false otherwise.
+    static constexpr int kSizeSteps = 64;
+    WC_D bool size_by_count(index_t c, index_t& size) const noexcept {
+        index_t cum = min_size;
+        size = min_size;
+        for (int k = 0; k < kSizeSteps && cum < c; ++k) {
+            const index_t next = grown(size);
+            if (next == size) return true;   // the same size from here on
+            size = next;
+            cum += size;
+        }
+        return cum >= c;
+    }
+    WC_D index_t grown(index_t size) const noexcept { return index_t(fminf(float(max_size), ceilf(float(size) * grow))); }
+
     WC_D Status append(handle_type* handle, const Value& value, index_t max_values_per_key) const noexcept {
-        using warpcore::detail::atomic_add;
+        return append_from(handle, warpcore::detail::atomic_load_relaxed(&handle->pack), value, max_values_per_key);
+    }
+
+    // append with the handle as the caller read it (`seen`); a failed CAS
+    // continues from the word it returned
+    WC_D Status append_from(handle_type* handle, std::uint64_t seen, const Value& value,
+                            index_t max_values_per_key) const noexcept {
         using warpcore::detail::atomic_cas;
         using warpcore::detail::atomic_exch;
         using LS = detail::LinkedListState;
         std::uint64_t* hp = &handle->pack;
         for (std::uint32_t reads = 0; reads < kMaxBlockedReads; ++reads) {
-            const handle_type cur{warpcore::detail::atomic_load_relaxed(hp)};
-            if (cur.is_blocking()) continue;
+            const handle_type cur{seen};
+            if (cur.is_blocking()) {   // another lane is linking a bucket: wait a little
+                __builtin_amdgcn_s_sleep(1);
+                seen = warpcore::detail::atomic_load_relaxed(hp);
+                continue;
+            }
             if (cur.is_uninitialized()) {
                 const handle_type blocked = handle_type::make(LS::blocking, head(), 0, 0);
-                if (atomic_cas(hp, cur.pack, blocked.pack) != cur.pack) continue;
-                const index_t alloc = atomic_add(next_free, min_size);
-                if (alloc + min_size <= capacity) {
+                seen = atomic_cas(hp, cur.pack, blocked.pack);
+                if (seen != cur.pack) continue;
+                const index_t alloc = allocate(min_size);
+                if (alloc != ~index_t(0)) {
                     buckets[alloc].value = value;
-                    __threadfence();
                     atomic_exch(hp, handle_type::make(LS::initialized, alloc, 1, 1).pack);
                     return Status::none();
                 }
@@ -131,21 +234,22 @@ struct BucketListStoreView {
             }
             if (cur.is_full()) return Status::out_of_memory();
             if (cur.value_count() == max_values_per_key) return Status::max_values_for_key_reached();
-            const index_t tail_size =
-                cur.value_count() <= min_size
-                    ? min_size
-                    : bucket_type::size_of(warpcore::detail::atomic_load_relaxed(&buckets[cur.bucket_index()].info));
+            index_t tail_size = min_size;
+            if (cur.value_count() > min_size && !size_by_count(cur.value_count(), tail_size))
+                tail_size = bucket_type::size_of(warpcore::detail::atomic_load_relaxed(&buckets[cur.bucket_index()].info));
             if (cur.num_values_tail() == tail_size) {
                 const handle_type blocked =
                     handle_type::make(LS::blocking, cur.bucket_index(), cur.value_count(), cur.num_values_tail());
-                if (atomic_cas(hp, cur.pack, blocked.pack) != cur.pack) continue;
-                const index_t new_size =
-                    index_t(fminf(float(max_size), ceilf(float(tail_size) * grow)));
-                const index_t alloc = atomic_add(next_free, new_size + 1);
-                if (alloc + new_size + 1 <= capacity) {
+                seen = atomic_cas(hp, cur.pack, blocked.pack);
+                if (seen != cur.pack) continue;
+                const index_t new_size = grown(tail_size);
+                const index_t alloc = allocate(new_size + 1);
+                if (alloc != ~index_t(0)) {
                     buckets[alloc + 1].value = value;
-                    atomic_exch(&buckets[alloc].info, bucket_type::make_info(cur.bucket_index(), new_size));
-                    __threadfence();
+                    const std::uint64_t link = bucket_type::make_info(cur.bucket_index(), new_size);
+                    index_t known = 0;
+                    if (size_by_count(cur.value_count() + 1, known)) atomic_exch(&buckets[alloc].info, link);
+                    else wait_for(atomic_exch(&buckets[alloc].info, link));   // appends will read this link
                     atomic_exch(hp, handle_type::make(LS::initialized, alloc, cur.value_count() + 1, 1).pack);
                     return Status::none();
                 }
@@ -156,9 +260,10 @@ struct BucketListStoreView {
             }
             const handle_type next = handle_type::make(LS::initialized, cur.bucket_index(), cur.value_count() + 1,
                                                        cur.num_values_tail() + 1);
-            if (atomic_cas(hp, cur.pack, next.pack) == cur.pack) {
+            seen = atomic_cas(hp, cur.pack, next.pack);
+            if (seen == cur.pack) {
                 const index_t j = cur.value_count() + 1 <= min_size ? cur.num_values_tail()
-                                                                                : cur.num_values_tail() + 1;
+                                                                    : cur.num_values_tail() + 1;
                 buckets[cur.bucket_index() + j].value = value;
                 return Status::none();
             }
@@ -233,13 +338,13 @@ public:
                                   index_type max_bucket_size = handle_type::max_bucket_size()) noexcept
         : status_(Status::not_initialized()), capacity_(max_capacity), bucket_grow_factor_(bucket_grow_factor),
           min_bucket_size_(min_bucket_size), max_bucket_size_(max_bucket_size), buckets_(nullptr),
-          next_free_bucket_(nullptr), is_copy_(false) {
+          counters_(nullptr), arenas_(arenas_for(max_capacity)), is_copy_(false) {
         if (capacity_ < handle_type::max_bucket_index() && bucket_grow_factor_ >= 1.0f && min_bucket_size_ >= 1 &&
             max_bucket_size_ >= min_bucket_size_ && max_bucket_size_ <= handle_type::max_bucket_size()) {
-            const std::size_t bytes = sizeof(bucket_type) * capacity_ + sizeof(index_type);
+            const std::size_t bytes = sizeof(bucket_type) * capacity_ + counter_bytes();
             if (warpcore::detail::available_gpu_memory() >= bytes &&
                 (capacity_ == 0 || hipMalloc(&buckets_, sizeof(bucket_type) * capacity_) == hipSuccess) &&
-                hipMalloc(&next_free_bucket_, sizeof(index_type)) == hipSuccess) {
+                hipMalloc(&counters_, counter_bytes()) == hipSuccess) {
                 status_ = Status::none();
                 init();
             } else {
@@ -252,32 +357,47 @@ public:
     WC_HD BucketListStore(const BucketListStore& o) noexcept
         : status_(o.status_), capacity_(o.capacity_), bucket_grow_factor_(o.bucket_grow_factor_),
           min_bucket_size_(o.min_bucket_size_), max_bucket_size_(o.max_bucket_size_), buckets_(o.buckets_),
-          next_free_bucket_(o.next_free_bucket_), is_copy_(true) {}
+          counters_(o.counters_), arenas_(o.arenas_), is_copy_(true) {}
     WC_H BucketListStore(BucketListStore&& o) noexcept
         : status_(o.status_), capacity_(o.capacity_), bucket_grow_factor_(o.bucket_grow_factor_),
           min_bucket_size_(o.min_bucket_size_), max_bucket_size_(o.max_bucket_size_), buckets_(o.buckets_),
-          next_free_bucket_(o.next_free_bucket_), is_copy_(o.is_copy_) {
+          counters_(o.counters_), arenas_(o.arenas_), is_copy_(o.is_copy_) {
         o.is_copy_ = true;
     }
     WC_HD ~BucketListStore() noexcept {
 #if !defined(__HIP_DEVICE_COMPILE__)
         if (!is_copy_) {
             if (buckets_ != nullptr) (void)hipFree(buckets_);
-            if (next_free_bucket_ != nullptr) (void)hipFree(next_free_bucket_);
+            if (counters_ != nullptr) (void)hipFree(counters_);
         }
 #endif
     }
 
     // storage.cuh:1063-1090: every cell becomes a header {head, min size};
-    // also empties the pool (next_free = 0)
+    // also empties the pool (every arena's counter = 0, by a fill kernel)
     WC_H void init(hipStream_t stream = 0) noexcept {
-        if (status_.has_any() || next_free_bucket_ == nullptr) return;
+        if (status_.has_any() || counters_ == nullptr) return;
         if (capacity_ != 0)
             kernels::fill<std::uint64_t><<<warpcore::detail::grid_for(capacity_, 1, 8), block_size(), 0, stream>>>(
                 reinterpret_cast<std::uint64_t*>(buckets_), capacity_ * (sizeof(bucket_type) / 8),
                 bucket_type::make_info(head(), min_bucket_size_));
-        (void)hipMemsetAsync(next_free_bucket_, 0, sizeof(index_type), stream);
+        kernels::fill<index_type><<<1, block_size(), 0, stream>>>(counters_, counter_bytes() / sizeof(index_type),
+                                                                  index_type(0));
         status_ = Status::none();
+    }
+
+    static constexpr index_type kMaxArenas = 256;
+    static constexpr index_type kMinArenaCells = index_type(1) << 16;
+    // arenas of a pool: the largest power of two up to kMaxArenas leaving
+    // every arena at least kMinArenaCells cells
+    WC_HD static constexpr index_type arenas_for(index_type cap) noexcept {
+        index_type a = 1;
+        while (a < kMaxArenas && cap / (2 * a) >= kMinArenaCells) a *= 2;
+        return a;
+    }
+    WC_HD index_type arenas() const noexcept { return arenas_; }
+    WC_HD std::size_t counter_bytes() const noexcept {
+        return arenas_ == 1 ? sizeof(index_type) : arenas_ * view_type::kCounterStride * sizeof(index_type);
     }
 
     WC_D status_type append(handle_type* handle, const value_type& value, index_type max_values_per_key) const
@@ -297,22 +417,29 @@ public:
     }
 
     WC_HD view_type view() const noexcept {
-        return view_type{buckets_, next_free_bucket_, capacity_, bucket_grow_factor_, min_bucket_size_,
-                         max_bucket_size_};
+        return view_type{buckets_, counters_, capacity_, bucket_grow_factor_, min_bucket_size_, max_bucket_size_,
+                         arenas_, capacity_ / arenas_};
     }
 
     WC_HD status_type status() const noexcept { return status_; }
     WC_HD index_type capacity() const noexcept { return capacity_; }
-    WC_H index_type bytes_total() const noexcept { return capacity_ * sizeof(bucket_type) + sizeof(index_type); }
+    WC_H index_type bytes_total() const noexcept { return capacity_ * sizeof(bucket_type) + counter_bytes(); }
+    // cells handed out: with one arena its counter, as the reference's
+    // next_free (it also counts the cells of failed allocations); with
+    // several, each arena's counter up to the arena's size (an arena a
+    // request did not fit counts as used up to its end)
     WC_H index_type next_free(hipStream_t stream = 0) const noexcept {
-        index_type n = 0;
-        if (next_free_bucket_ == nullptr) return 0;
-        (void)hipMemcpyAsync(&n, next_free_bucket_, sizeof(index_type), hipMemcpyDeviceToHost, stream);
+        if (counters_ == nullptr) return 0;
+        std::vector<index_type> c(counter_bytes() / sizeof(index_type));
+        (void)hipMemcpyAsync(c.data(), counters_, counter_bytes(), hipMemcpyDeviceToHost, stream);
         (void)hipStreamSynchronize(stream);
+        if (arenas_ == 1) return c[0];
+        const view_type v = view();
+        index_type n = 0;
+        for (index_type a = 0; a < arenas_; ++a) n += std::min(c[a * view_type::kCounterStride], v.arena_size(a));
         return n;
     }
-    // storage.cuh:1370-1380 (next_free counts cells handed out, also those
-    // of failed allocations)
+    // storage.cuh:1370-1380
     WC_H float load_factor(hipStream_t stream = 0) const noexcept {
         return float(next_free(stream)) / float(capacity());
     }
@@ -331,7 +458,8 @@ private:
     index_type min_bucket_size_;
     index_type max_bucket_size_;
     bucket_type* buckets_;
-    index_type* next_free_bucket_;
+    index_type* counters_;
+    index_type arenas_;
     bool is_copy_;
 };
 
@@ -390,9 +518,51 @@ struct BucketListView {
     }
 };
 
+// Key pass of a bucket-list insert on the lane-owner kernel: finds or
+// claims each key's slot in the key table without touching its handle and
+// records the slot of every input pair (slot_of; it stays ~0 where the key
+// failed) for the append pass.
+template<class KeyView>
+struct BucketListKeyCore : KeyView {
+    std::uint32_t* slot_of;
+    template<class Join>
+    WC_D void finish_insert(index_t, const typename KeyView::value_type&, bool, Status&, Join&) const noexcept {}
+    WC_D void on_slot(index_t slot, index_t pos) const noexcept { slot_of[pos] = std::uint32_t(slot); }
+};
+
 }  // namespace detail
 
 namespace kernels {
+
+// Append pass of a bucket-list insert: one lane per pair, the pair's value
+// appended to the handle of the slot the key pass recorded (the appends of
+// a wavefront are independent); the pair's status is the key pass's
+// (key_status, or none) + the append's (bucket_list_hash_table.cuh:234-339
+// per pair).
+template<class View, class StatusHandler>
+__global__ __launch_bounds__(256) void bl_append(const typename View::value_type* __restrict__ values_in,
+                                                 index_t num_in, const std::uint32_t* __restrict__ slot_of,
+                                                 const Status* __restrict__ key_status, View view,
+                                                 typename StatusHandler::base_type* __restrict__ status_out) {
+    using handle_type = typename View::handle_type;
+    warpcore::detail::DeferredJoin join;
+    const index_t stride = index_t(gridDim.x) * blockDim.x;
+    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < num_in; i += stride) {
+        const std::uint32_t slot = slot_of[i];
+        Status st = key_status != nullptr ? key_status[i] : Status::none();
+        if (slot != ~0u) {
+            handle_type* h = &view.keys.slots[slot].value;
+            const std::uint64_t seen = warpcore::detail::atomic_load_relaxed(&h->pack);
+            Status a = Status::max_values_for_key_reached();
+            if (handle_type{seen}.value_count() < view.max_values)
+                a = view.store.append_from(h, seen, values_in[i], view.max_values);
+            if (a.has_any()) join(a);
+            st = st + a;
+        }
+        StatusHandler::handle(st, status_out, i);
+    }
+    join.flush(view.keys.status_word());
+}
 
 template<class View, class StatusHandler>
 __global__ __launch_bounds__(256) void bl_insert(const typename View::key_type* __restrict__ keys_in,
@@ -542,6 +712,7 @@ public:
     WC_HD ~BucketListHashTable() noexcept {
 #if !defined(__HIP_DEVICE_COMPILE__)
         if (!is_copy_ && scan_parts_ != nullptr) (void)hipFree(scan_parts_);
+        if (!is_copy_ && scratch_ != nullptr) (void)hipFree(scratch_);
 #endif
     }
 
@@ -602,6 +773,41 @@ public:
                      typename StatusHandler::base_type* status_out = nullptr) noexcept {
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!hash_table_.is_initialized_ || num_in == 0) return;
+        using key_view = typename hash_table_type::view_type;
+        using key_core = detail::BucketListKeyCore<key_view>;
+        if constexpr (warpcore::detail::fixed_step_scheme<typename key_view::probing_scheme_type>::value &&
+                      warpcore::detail::owner_insert_capable<key_core>()) {
+            // key pass on the lane-owner kernel, then one lane per pair appends
+            // (in chunks of the table's scratch: each pair's slot and key status)
+            constexpr index_type chunk = index_type(1) << 24;
+            const index_type cap = num_in < chunk ? num_in : chunk;
+            if (warpcore::detail::owner_inserts() && reserve_scratch(cap)) {
+                constexpr bool statuses = !std::is_same<StatusHandler, status_handlers::ReturnNothing>::value;
+                std::uint32_t* slot_of = scratch_;
+                Status* key_status = reinterpret_cast<Status*>(scratch_ + scratch_cap_);
+                for (index_type first = 0; first < num_in; first += cap) {
+                    const index_type n = num_in - first < cap ? num_in - first : cap;
+                    storage::kernels::fill<std::uint32_t>
+                        <<<warpcore::detail::grid_for(n, 1, 8), block_size(), 0, stream>>>(slot_of, n, ~0u);
+                    const key_core core{hash_table_.view(), slot_of};
+                    Status* ks = statuses && status_out != nullptr ? key_status : nullptr;
+                    if (ks != nullptr)
+                        warpcore::detail::launch_insert_owner<key_core, status_handlers::ReturnStatus, false>(
+                            keys_in + first, nullptr, n, core, probing_length, ks, stream, nullptr);
+                    else
+                        warpcore::detail::launch_insert_owner<key_core, status_handlers::ReturnNothing, false>(
+                            keys_in + first, nullptr, n, core, probing_length, nullptr, stream, nullptr);
+                    typename StatusHandler::base_type* out = nullptr;
+                    if constexpr (statuses)
+                        if (status_out != nullptr) out = status_out + first;
+                    kernels::bl_append<view_type, StatusHandler>
+                        <<<warpcore::detail::grid_for(n, 1, 32), block_size(), 0, stream>>>(
+                            values_in + first, n, slot_of, ks, view(), out);
+                }
+                if (stream == 0) (void)hipStreamSynchronize(stream);
+                return;
+            }
+        }
         kernels::bl_insert<view_type, StatusHandler>
             <<<warpcore::detail::grid_for(num_in, cg_size()), block_size(), 0, stream>>>(
                 keys_in, values_in, num_in, view(), probing_length, status_out);
@@ -747,6 +953,23 @@ private:
     index_type max_values_per_key_;
     bool is_copy_;
     index_type* scan_parts_;   // scratch of the two-pass retrieve's prefix sum
+    // scratch of the two-pass insert (per pair of a chunk: its key's slot and
+    // its key status), grown on demand; copies of the table do not own it
+    std::uint32_t* scratch_ = nullptr;
+    index_type scratch_cap_ = 0;
+    WC_H bool reserve_scratch(index_type n) noexcept {
+        if (n <= scratch_cap_) return true;
+        if (is_copy_) return false;
+        if (scratch_ != nullptr) (void)hipFree(scratch_);
+        scratch_ = nullptr;
+        scratch_cap_ = 0;
+        if (hipMalloc(&scratch_, 2 * sizeof(std::uint32_t) * n) != hipSuccess) {
+            scratch_ = nullptr;
+            return false;
+        }
+        scratch_cap_ = n;
+        return true;
+    }
 };
 
 }  // namespace warpcore
