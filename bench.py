@@ -700,6 +700,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (PMC calibration)")
+    only = {"hashset": run_hashset, "bloom": run_bloom, "mvht": run_mvht, "bucketlist": run_bucketlist,
+            "batched": run_batched, "serving": run_serving}
+    ap.add_argument("--only", choices=sorted(only),
+                    help="run this one structure's workload alone and print its result as JSON "
+                         "(for profiling runs; no benchmark line)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -709,6 +714,9 @@ def main():
         sys.exit(2)
 
     g = Gpu()
+    if args.only:
+        print(json.dumps({args.only: only[args.only](g)}), flush=True)
+        return
     log(f"[bench] svht u64/u64 2^{args.log2_keys} keys, steps={args.steps} warmup={args.warmup}")
     r64 = run_svht(g, 64, 64, 8, args.log2_keys, args.steps, args.warmup)
     log(f"[bench] u64: {r64}")

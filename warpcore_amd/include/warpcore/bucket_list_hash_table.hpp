@@ -170,7 +170,9 @@ struct BucketListStoreView {
     // A new bucket's link (info word) read by appends must be visible
     // before the handle that points to it: an append that cannot size the
     // tail from the count (size_by_count) reads the tail's link with a
-    // device-scope atomic load once it sees the handle. Both are written
+    // device-scope atomic load once it sees the handle. (Links that the
+    // count sizes are read by later launches only and are plain stores.)
+    // Both are written
     // with device-scope atomics, which the memory side performs; the link's
     // exchange returns its old word, and waiting for that word before the
     // handle's exchange is issued orders the two at the memory side for
@@ -248,8 +250,10 @@ struct BucketListStoreView {
                     buckets[alloc + 1].value = value;
                     const std::uint64_t link = bucket_type::make_info(cur.bucket_index(), new_size);
                     index_t known = 0;
-                    if (size_by_count(cur.value_count() + 1, known)) atomic_exch(&buckets[alloc].info, link);
-                    else wait_for(atomic_exch(&buckets[alloc].info, link));   // appends will read this link
+                    if (size_by_count(cur.value_count() + 1, known))
+                        buckets[alloc].info = link;   // read only by later launches: a plain store
+                    else
+                        wait_for(atomic_exch(&buckets[alloc].info, link));   // appends will read this link
                     atomic_exch(hp, handle_type::make(LS::initialized, alloc, cur.value_count() + 1, 1).pack);
                     return Status::none();
                 }
@@ -275,6 +279,43 @@ struct BucketListStoreView {
     template<class Func, unsigned G>
     WC_D void for_each(Func&& f, const handle_type& handle, const tile<G>& group) const noexcept {
         for_each_strided(f, handle, group.thread_rank(), G);
+    }
+
+    // One lane's walk of a handle's buckets, tail first, put(value, j) for
+    // j < count. Bucket sizes come from the value counts (size_by_count), so
+    // each bucket costs one round trip: its link and its values are loaded
+    // together, B values at a time, before any of them is put.
+    template<unsigned B, class Put>
+    WC_D void walk_lane(const handle_type& handle, Put& put) const noexcept {
+        if ((!handle.is_initialized() && !handle.is_full()) || handle.bucket_index() == head()) return;
+        const index_t count = handle.value_count();
+        index_t bucket = handle.bucket_index();
+        index_t size = handle.num_values_tail();   // values to read in this bucket
+        index_t base = 0;                          // values before this bucket, in walk order
+        while (true) {
+            const bool headed = count - base > min_size;   // not the first, headerless bucket
+            const index_t off = headed ? 1 : 0;
+            if (size == 0 || bucket + off + size > capacity) return;   // not on a well-formed chain
+            const std::uint64_t info = headed ? buckets[bucket].info : 0;
+            for (index_t p = 0; p < size; p += B) {
+                Value v[B];
+#pragma unroll
+                for (unsigned b = 0; b < B; ++b)
+                    if (p + b < size) v[b] = buckets[bucket + off + p + b].value;
+#pragma unroll
+                for (unsigned b = 0; b < B; ++b)
+                    if (p + b < size) put(v[b], base + p + b);
+            }
+            base += size;
+            if (base >= count || !headed) return;
+            bucket = bucket_type::previous_of(info);
+            if (bucket == head() || bucket >= capacity) return;
+            if (count - base <= min_size) {
+                size = min_size;
+            } else if (!size_by_count(count - base, size)) {
+                size = bucket_type::size_of(buckets[bucket].info);
+            }
+        }
     }
 
     // Lane `rank` of `group_size` lanes takes positions rank, rank + group_size,
@@ -641,6 +682,38 @@ __global__ __launch_bounds__(256) void bl_retrieve(const typename View::key_type
     join.flush(view.status_word());
 }
 
+// Snapshot retrieve, second step: counts[i] = value count of the handle
+// the key lookup wrote to handles[i] (0 for a key it did not find)
+template<class Handle>
+__global__ __launch_bounds__(256) void bl_counts(const std::uint64_t* __restrict__ handles, index_t num_in,
+                                                 index_t* __restrict__ counts) {
+    const index_t stride = index_t(gridDim.x) * blockDim.x;
+    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < num_in; i += stride)
+        counts[i] = Handle{handles[i]}.value_count();
+}
+
+// Snapshot retrieve, last step: one lane per key walks the buckets of the
+// handle in begin_io[i] (kept there by the lookup), replaces it by the key's
+// first output position (end - count) and writes the values tail bucket
+// first. A lane per key keeps 64 chains in flight per wavefront.
+template<class View>
+__global__ __launch_bounds__(256) void bl_walk(index_t num_in, index_t* __restrict__ begin_io,
+                                               const index_t* __restrict__ end_in,
+                                               typename View::value_type* __restrict__ values_out, View view) {
+    using V = typename View::value_type;
+    using handle_type = typename View::handle_type;
+    const index_t stride = index_t(gridDim.x) * blockDim.x;
+    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < num_in; i += stride) {
+        const handle_type h{begin_io[i]};
+        const index_t count = h.value_count();
+        const index_t begin = end_in[i] - count;
+        begin_io[i] = begin;
+        V* out = values_out + begin;
+        auto put = [=](const V& v, index_t j) { if (j < count) out[j] = v; };
+        view.store.template walk_lane<4>(h, put);
+    }
+}
+
 // sum of the value counts of all keys (bucket_list_hash_table.cuh:805-822)
 template<class View>
 __global__ __launch_bounds__(256) void bl_total_values(index_t* __restrict__ num_out, View view) {
@@ -830,6 +903,13 @@ public:
             num_out = 0;
             return;
         }
+        if constexpr (warpcore::detail::owner_capable<typename hash_table_type::view_type>()) {
+            if (values_out != nullptr && scan_parts_ != nullptr && warpcore::detail::owner_lookups()) {
+                retrieve_snapshot<StatusHandler>(keys_in, num_in, begin_offsets_out, end_offsets_out, values_out,
+                                                 num_out, stream, probing_length, status_out);
+                return;
+            }
+        }
         num_values(keys_in, num_in, num_out, end_offsets_out, stream, probing_length);
         if (values_out != nullptr) {
             if (scan_parts_ == nullptr) {
@@ -852,6 +932,34 @@ public:
             join_status(Status::dry_run(), stream);
         }
         if (stream == 0) (void)hipStreamSynchronize(stream);
+    }
+
+    // retrieve with the lane-owner key lookup: each key's handle is read once
+    // (into begin_offsets_out), its count becomes end_offsets_out, the scan
+    // makes the counts end offsets, and one lane per key walks its buckets.
+    // Counts and values come from the one handle read, so the two never
+    // disagree (no invalid_phase_overlap); statuses are the lookup's.
+    template<class StatusHandler>
+    WC_H void retrieve_snapshot(const key_type* keys_in, index_type num_in, index_type* begin_offsets_out,
+                                index_type* end_offsets_out, value_type* values_out, index_type& num_out,
+                                hipStream_t stream, index_type probing_length,
+                                typename StatusHandler::base_type* status_out) const noexcept {
+        static_assert(sizeof(handle_type) == sizeof(index_type), "handles are kept in the begin offsets");
+        // a key the lookup does not find keeps the word 0: an uninitialised handle of no values
+        storage::kernels::fill<index_type><<<warpcore::detail::grid_for(num_in, 1, 8), block_size(), 0, stream>>>(
+            begin_offsets_out, num_in, index_type(0));
+        warpcore::detail::launch_retrieve_owner<typename hash_table_type::view_type, StatusHandler>(
+            keys_in, num_in, reinterpret_cast<handle_type*>(begin_offsets_out), hash_table_.view(), probing_length,
+            status_out, stream);
+        kernels::bl_counts<handle_type><<<warpcore::detail::grid_for(num_in, 1, 8), block_size(), 0, stream>>>(
+            reinterpret_cast<const std::uint64_t*>(begin_offsets_out), num_in, end_offsets_out);
+        warpcore::detail::device_scan(end_offsets_out, end_offsets_out, num_in, scan_parts_, true, stream);
+        kernels::bl_walk<view_type><<<warpcore::detail::grid_for(num_in, 1, 32), block_size(), 0, stream>>>(
+            num_in, begin_offsets_out, end_offsets_out, values_out, view());
+        num_out = 0;
+        (void)hipMemcpyAsync(&num_out, end_offsets_out + (num_in - 1), sizeof(index_type), hipMemcpyDeviceToHost,
+                             stream);
+        (void)hipStreamSynchronize(stream);
     }
 
     // :460-488; num_values_out is index_type& (see MultiValueHashTable)
