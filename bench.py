@@ -62,16 +62,18 @@ def algorithmic_bytes(op: str, kb: int, vb: int) -> int:
 KERNEL_REVISION = "owner-3"
 
 
-def read_traffic(family: str, log2n: int = 28):
+def read_traffic(family: str, log2n: int = 28, table: str = "u64"):
     """HBM bytes per launch of kernel `family` (insert_owner, retrieve_owner,
-    ...) on this workload (2^28 keys), measured by rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, each scaled by a copy kernel of known size), if
+    ...) on this workload (2^28 keys; `table` "u64" for SVHT<u64,u64> cg 8,
+    "u32" for SVHT<u32,u32> cg 16), measured by rocprofv3 --pmc (FETCH_SIZE
+    + WRITE_SIZE, each scaled by a copy kernel of known size), if
     profiles/svht_u64_lf09_traffic.md holds a figure for that kernel at this
     KERNEL_REVISION; else None (a figure of another kernel is never used)."""
     f = REPO / "profiles" / "svht_u64_lf09_traffic.md"
     if log2n != 28 or not f.exists():
         return None
-    m = re.search(rf"^traffic_bytes_per_launch\.{family}@{KERNEL_REVISION}\s*=\s*([0-9.eE+]+)", f.read_text(),
+    key = family if table == "u64" else f"{table}\\.{family}"
+    m = re.search(rf"^traffic_bytes_per_launch\.{key}@{KERNEL_REVISION}\s*=\s*([0-9.eE+]+)", f.read_text(),
                   re.M)
     return float(m.group(1)) if m else None
 
@@ -793,6 +795,17 @@ def main():
             lr[f"{op}_lines_per_op"] = round(lines_per_op, 3)
             lr[f"{op}_frac"] = round(n / (kt[op] * 1e-3) * lines_per_op / 1e9 / lr["glines_s"], 4)
     log(f"[bench] line roofline: {lr}")
+    if "u32u32_cg16" in extras:   # the same for the u32 table (2.4 GB of 8-byte slots)
+        lr32 = run_line_roofline(g, r32["table_bytes"])
+        for op in ("retrieve", "insert"):
+            tr = read_traffic(fam[op], args.log2_keys, "u32")
+            if tr:
+                lines_per_op = tr / LINE / r32["n"]
+                lr32[f"{op}_lines_per_op"] = round(lines_per_op, 3)
+                lr32[f"{op}_frac"] = round(r32["n"] / (r32[f"{op}_ms"] * 1e-3) * lines_per_op / 1e9 /
+                                           lr32["glines_s"], 4)
+        extras["u32u32_cg16"]["line_roofline"] = lr32
+        log(f"[bench] u32 line roofline: {lr32}")
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline ...")
