@@ -6,7 +6,8 @@ for word against the oracle. Sizes: SVHT<u64,u64> 2^28 keys at load factor
 0.9 (4.8 GB), SVHT<u32,u32> 2^20 keys at capacity 2^20/0.8,
 CountingHashTable<u64> over 2^28 Zipf(1.1) occurrences of 2^24 keys,
 MultiValueHashTable<u32,u32> 2^26 pairs with geometric multiplicity of mean
-8, HashSet<u32> 2^27 keys, BloomFilter 2^30 bits with k=8 and 2^27 keys."""
+8, BucketListHashTable<u32,u32> 2^27 values on 2^24 keys, HashSet<u32> 2^27
+keys, BloomFilter 2^30 bits with k=8 and 2^27 keys."""
 import ctypes
 import math
 
@@ -151,6 +152,42 @@ def test_full_mvht_geometric_multisets(cuda):
     per_key = (end - beg)
     rk = torch.repeat_interleave(kout, per_key)
     # (key, value) multisets: pack each pair into one 64-bit word and sort
+    pack = lambda k, v: (k.to(torch.int64) & 0xFFFFFFFF) << 32 | (v.to(torch.int64) & 0xFFFFFFFF)
+    a = torch.sort(pack(rk, vout)).values
+    b = torch.sort(pack(dk, dv)).values
+    assert torch.equal(a, b)
+
+
+def test_full_bucketlist_multisets(cuda):
+    """2^24 keys, each 8 times (2^27 pairs, shuffled), pool 2^28 cells in 256
+    arenas, buckets growing by 1.1 from 1 value: every key's buckets are
+    1, 2, 3 and 4 values long (13 cells with the three headers), so the
+    pool usage is exact; the (key, value) multisets equal the inputs'."""
+    from warpcore_amd.tables import BucketListHashTable
+    d, mult = 1 << 24, 8
+    n = d * mult
+    print("[full] bucketlist u32 2^27 values on 2^24 keys", flush=True)
+    ukeys = _gen_unique(d, 32, 0x6000)
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0xC6)
+    dk = ukeys.repeat(mult)[torch.randperm(n, device="cuda", generator=gen)].contiguous()
+    dv = _gen_uniform(n, 32, 0xC7)
+    t = BucketListHashTable(math.ceil(d / 0.9), 1 << 28, 32, 32, grow_factor=1.1, min_bucket_size=1, cg_size=8)
+    st = torch.empty(n, dtype=torch.int32, device="cuda")
+    t.insert(dk, dv, status_out=st)
+    assert int((st == S_NONE).sum()) == d                # one new key per distinct key
+    assert int((st & ~16).count_nonzero()) == 0          # the rest duplicate_key only
+    assert t.num_keys() == d and t.num_values() == n
+    assert t.value_cells_used() == 13 * d
+    kout = torch.empty(d, dtype=torch.int32, device="cuda")
+    beg = torch.empty(d, dtype=torch.int64, device="cuda")
+    end = torch.empty(d, dtype=torch.int64, device="cuda")
+    vout = torch.empty(n, dtype=torch.int32, device="cuda")
+    got_k, got_v = t.retrieve_all(kout, beg, end, vout)
+    assert got_k == d and got_v == n
+    per_key = end - beg
+    assert bool((per_key == mult).all()) and int(beg[0]) == 0 and bool((beg[1:] == end[:-1]).all())
+    rk = torch.repeat_interleave(kout, per_key)
     pack = lambda k, v: (k.to(torch.int64) & 0xFFFFFFFF) << 32 | (v.to(torch.int64) & 0xFFFFFFFF)
     a = torch.sort(pack(rk, vout)).values
     b = torch.sort(pack(dk, dv)).values
