@@ -703,7 +703,8 @@ def main():
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (PMC calibration)")
     only = {"hashset": run_hashset, "bloom": run_bloom, "mvht": run_mvht, "bucketlist": run_bucketlist,
-            "batched": run_batched, "serving": run_serving}
+            "batched": run_batched, "serving": run_serving,
+            "counting": lambda g: run_counting(g, 28, 24, 1.1, 5, 1)}
     ap.add_argument("--only", choices=sorted(only),
                     help="run this one structure's workload alone and print its result as JSON "
                          "(for profiling runs; no benchmark line)")

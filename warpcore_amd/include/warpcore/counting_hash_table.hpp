@@ -64,9 +64,13 @@ struct CountingView {
     // ends where `inc` single increments would: each of those adds one only
     // while the count is below max_count, so the count becomes
     // min(count + inc, max_count) and never wraps (no index_overflow, which
-    // a single add of `inc` past the type's maximum would raise).
+    // a single add of `inc` past the type's maximum would raise). All but the
+    // first of those occurrences find the key stored, so the table's status
+    // joins duplicate_key, as the reference does for each duplicate insert.
     template<class Join>
     WC_D void finish_insert(index_type slot, const value_type& inc, bool, Status& status, Join& join) const noexcept {
+        if constexpr (AddValue)
+            if (inc > value_type(1)) join(Status::duplicate_key());
         const value_type add = AddValue ? inc : value_type(1);
         value_type* v = &(base.slots[slot].value);
         if constexpr (sizeof(value_type) == 8) {
@@ -299,24 +303,6 @@ __global__ __launch_bounds__(256) void count_combine(const K* __restrict__ keys_
     }
 }
 
-// Join duplicate_key for keys that occurred more than once in the batch
-// and were stored (the reference joins it for every duplicate occurrence).
-template<class V>
-__global__ __launch_bounds__(256) void count_join_duplicates(const V* __restrict__ counts,
-                                                             const Status* __restrict__ entry_status,
-                                                             const index_t* __restrict__ num_entries,
-                                                             Status* __restrict__ word) {
-    detail::DeferredJoin join;
-    const index_t n = *num_entries;
-    const index_t stride = index_t(gridDim.x) * blockDim.x;
-    for (index_t e = index_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += stride) {
-        const Status s = entry_status[e];
-        if (counts[e] > 1 && !s.has_any(Status::flags_except(Status::duplicate_key() + Status::index_overflow())))
-            join(Status::duplicate_key());
-    }
-    join.flush(word);
-}
-
 // Per occurrence: its entry's status for the first occurrence of a key;
 // for the others duplicate_key where the entry was stored (none or
 // duplicate), else the entry's status (the table was full, ...).
@@ -521,18 +507,19 @@ public:
             combine(keys_in + first, n, buf, stream);
             if constexpr (detail::fixed_step_scheme<ProbingScheme>::value) {
                 // the insert reads the entry count on the device: no host round trip
-                detail::launch_insert_refill<add_view_type, status_handlers::ReturnStatus,
-                                             detail::refill_insert_keys<unsigned(cg_size())>(), true>(
-                    buf.keys, buf.counts, n, add_view(), probing_length, buf.status, stream, buf.num);
-                finish_combined<StatusHandler>(buf, first, n, n, stream, status_out);
+                insert_entries<StatusHandler>(buf, n, stream, probing_length, status_out);
             } else {
                 index_type entries = 0;
                 (void)hipMemcpyAsync(&entries, buf.num, sizeof(index_type), hipMemcpyDeviceToHost, stream);
                 (void)hipStreamSynchronize(stream);
-                insert_each<add_view_type, status_handlers::ReturnStatus, true>(
-                    buf.keys, buf.counts, entries, add_view(), stream, probing_length, buf.status);
-                finish_combined<StatusHandler>(buf, first, n, entries, stream, status_out);
+                if (entry_statuses<StatusHandler>(status_out))
+                    insert_each<add_view_type, status_handlers::ReturnStatus, true>(
+                        buf.keys, buf.counts, entries, add_view(), stream, probing_length, buf.status);
+                else
+                    insert_each<add_view_type, status_handlers::ReturnNothing, true>(
+                        buf.keys, buf.counts, entries, add_view(), stream, probing_length, nullptr);
             }
+            finish_combined<StatusHandler>(buf, first, n, stream, status_out);
         }
     }
 
@@ -551,10 +538,8 @@ public:
             num_in >= (index_type(1) << 31))
             return;
         combine(keys_in, num_in, buf, stream);
-        detail::launch_insert_refill<add_view_type, status_handlers::ReturnStatus,
-                                     detail::refill_insert_keys<unsigned(cg_size())>(), true>(
-            buf.keys, buf.counts, num_in, add_view(), probing_length, buf.status, stream, buf.num);
-        finish_combined<StatusHandler>(buf, 0, num_in, num_in, stream, status_out);
+        insert_entries<StatusHandler>(buf, num_in, stream, probing_length, status_out);
+        finish_combined<StatusHandler>(buf, 0, num_in, stream, status_out);
     }
 
 private:
@@ -577,15 +562,33 @@ private:
                                                                       buf.pos, buf.num, buf.hot);
     }
 
-    // table-status join of duplicates and the per-occurrence statuses of
-    // occurrences [first, first + n) from the entries' statuses
-    // (max_entries bounds the entry count read on the device)
+    // whether the entries' statuses are needed (per-occurrence statuses asked for)
     template<class StatusHandler>
-    WC_H void finish_combined(const CombineBuffers& buf, index_type first, index_type n, index_type max_entries,
-                              hipStream_t stream, typename StatusHandler::base_type* status_out) noexcept {
-        kernels::count_join_duplicates<value_type>
-            <<<detail::grid_for(max_entries, 1, 4), block_size(), 0, stream>>>(buf.counts, buf.status, buf.num,
-                                                                              base_table_.status_);
+    WC_HD static bool entry_statuses(const typename StatusHandler::base_type* status_out) noexcept {
+        return !std::is_same<StatusHandler, status_handlers::ReturnNothing>::value && status_out != nullptr;
+    }
+
+    // insert of the combined entries of one pass (at most n; the kernel
+    // reads their number on the device), writing the entries' statuses only
+    // when per-occurrence statuses follow
+    template<class StatusHandler>
+    WC_H void insert_entries(const CombineBuffers& buf, index_type n, hipStream_t stream, index_type probing_length,
+                             typename StatusHandler::base_type* status_out) noexcept {
+        constexpr int U = detail::refill_insert_keys<unsigned(cg_size())>();
+        if (entry_statuses<StatusHandler>(status_out))
+            detail::launch_insert_refill<add_view_type, status_handlers::ReturnStatus, U, true>(
+                buf.keys, buf.counts, n, add_view(), probing_length, buf.status, stream, buf.num);
+        else
+            detail::launch_insert_refill<add_view_type, status_handlers::ReturnNothing, U, true>(
+                buf.keys, buf.counts, n, add_view(), probing_length, nullptr, stream, buf.num);
+    }
+
+    // per-occurrence statuses of occurrences [first, first + n) from the
+    // entries' statuses (the table's status joined duplicate_key in the
+    // insert, for every entry of more than one occurrence)
+    template<class StatusHandler>
+    WC_H void finish_combined(const CombineBuffers& buf, index_type first, index_type n, hipStream_t stream,
+                              typename StatusHandler::base_type* status_out) noexcept {
         if constexpr (!std::is_same<StatusHandler, status_handlers::ReturnNothing>::value)
             if (status_out != nullptr)
                 kernels::count_expand<StatusHandler><<<detail::grid_for(n, 1, 8), block_size(), 0, stream>>>(
