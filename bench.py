@@ -713,7 +713,7 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on exactly one GPU: the hash-table hot path has no multi-GPU "
-              "path in this build (sharding is described in DESIGN.md).", file=sys.stderr)
+              "path (tables would shard by key hash, one process per GPU).", file=sys.stderr)
         sys.exit(2)
 
     g = Gpu()
