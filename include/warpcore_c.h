@@ -536,6 +536,10 @@ WC_API uint32_t wc_bucketlist_u64u64_cg_size(void* table);
  *                key results and statuses as SingleValueHashTable's
  *   set_launch / get_launch  keys per work item, blocks per CU, threads per
  *                block (64..512, a multiple of 64)
+ *   set_lds_insert  inserts into the tables of at most 64 KiB of slots run
+ *                in LDS, one block per table (1, the default) or through the
+ *                work items like the others (0); same results and statuses
+ *   lds_tables   how many tables the LDS insert takes
  *   tune         autotuner: tables are put in classes by requested capacity
  *                (bands [4^k, 4^(k+1))) and fill (keys / requested capacity:
  *                < 0.6, < 0.7, < 0.8, >= 0.8); every configuration is timed
@@ -567,6 +571,8 @@ WC_API int wc_batch_u32u32_init(void* batch, void* stream);
 WC_API int wc_batch_u32u32_insert(void* batch, const uint32_t* keys_in, const uint32_t* values_in, const uint64_t* offsets, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_batch_u32u32_retrieve(void* batch, const uint32_t* keys_in, const uint64_t* offsets, uint64_t num_in, uint32_t* values_out, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_batch_u32u32_set_launch(void* batch, uint64_t item_keys, uint32_t blocks_per_cu, uint32_t block_size);
+WC_API int wc_batch_u32u32_set_lds_insert(void* batch, int on);
+WC_API uint64_t wc_batch_u32u32_lds_tables(void* batch);
 WC_API int wc_batch_u32u32_get_launch(void* batch, uint64_t* launch_out);
 WC_API int wc_batch_u32u32_tune(void* batch, const uint32_t* keys_in, const uint32_t* values_in, const uint64_t* offsets, uint64_t num_in, const uint32_t* queries, const uint64_t* query_offsets, uint64_t num_queries, uint32_t* values_out, void* stream, uint64_t* launch_out, double* ms_out);
 WC_API int wc_batch_u32u32_tune_report(void* batch, uint64_t* rows, uint64_t max_rows, double* totals);
@@ -588,6 +594,8 @@ WC_API int wc_batch_u64u64_init(void* batch, void* stream);
 WC_API int wc_batch_u64u64_insert(void* batch, const uint64_t* keys_in, const uint64_t* values_in, const uint64_t* offsets, uint64_t num_in, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_batch_u64u64_retrieve(void* batch, const uint64_t* keys_in, const uint64_t* offsets, uint64_t num_in, uint64_t* values_out, void* stream, uint64_t probing_length, uint32_t* status_out);
 WC_API int wc_batch_u64u64_set_launch(void* batch, uint64_t item_keys, uint32_t blocks_per_cu, uint32_t block_size);
+WC_API int wc_batch_u64u64_set_lds_insert(void* batch, int on);
+WC_API uint64_t wc_batch_u64u64_lds_tables(void* batch);
 WC_API int wc_batch_u64u64_get_launch(void* batch, uint64_t* launch_out);
 WC_API int wc_batch_u64u64_tune(void* batch, const uint64_t* keys_in, const uint64_t* values_in, const uint64_t* offsets, uint64_t num_in, const uint64_t* queries, const uint64_t* query_offsets, uint64_t num_queries, uint64_t* values_out, void* stream, uint64_t* launch_out, double* ms_out);
 WC_API int wc_batch_u64u64_tune_report(void* batch, uint64_t* rows, uint64_t max_rows, double* totals);

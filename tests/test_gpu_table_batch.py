@@ -117,6 +117,58 @@ def test_table_batch_matches_plain_launches(cuda):
         np.testing.assert_array_equal(got[q], from_dev(pout, np.uint32))
 
 
+@pytest.mark.parametrize("kb", [32, 64])
+def test_table_batch_lds_insert_matches_work_items(cuda, kb):
+    """The LDS insert of the small tables against the work-item kernel on
+    the same batch: per-table insert statuses (sorted: which occurrence of a
+    repeated key gets none is open), every lookup result and status, and the
+    per-table status words are equal; some tables are filled past capacity
+    and some keys are invalid."""
+    from warpcore_amd.tables import TableBatch
+    caps, keys, vals, queries, off, qoff = _workload(400, kb, seed=23, lo=6, hi=15)
+    for t in range(0, 400, 7):   # repeats: the value is the same for both copies
+        keys[t] = np.concatenate([keys[t], keys[t][: len(keys[t]) // 4]])
+        vals[t] = np.concatenate([vals[t], vals[t][: len(vals[t]) // 4]])
+    overfilled = [t for t in range(3, 400, 11) if t % 7 != 0]   # (with repeats the statuses depend on the order)
+    for t in overfilled:         # more keys than slots
+        extra = unique_keys(int(caps[t]) * 2, kb, seed=900 + t, exclude=keys[t])
+        keys[t] = np.concatenate([keys[t], extra])
+        vals[t] = np.concatenate([vals[t], extra])
+    for t in range(5, 400, 13):  # invalid keys (EMPTY and TOMBSTONE)
+        bad = np.array([0, (1 << kb) - 1], dtype(kb))
+        keys[t] = np.concatenate([keys[t], bad])
+        vals[t] = np.concatenate([vals[t], bad])
+    off = np.concatenate([[0], np.cumsum([len(k) for k in keys])]).astype(np.uint64)
+    K, V, Q = (to_dev(np.concatenate(x)) for x in (keys, vals, queries))
+    res = []
+    for lds in (True, False):
+        b = TableBatch(caps, kb, kb)
+        b.set_configs(np.random.default_rng(29).integers(0, len(b.configs()), len(caps)))
+        b.set_lds_insert(lds)
+        if lds:
+            assert 0 < b.lds_tables() < len(caps)
+        st = dev_empty(K.numel(), 32, 0x77)
+        b.insert(K, V, to_dev(off), status_out=st)
+        out, qst = dev_empty(Q.numel(), kb, 0xAB), dev_empty(Q.numel(), 32, 0x77)
+        b.retrieve(Q, to_dev(qoff), out, status_out=qst)
+        res.append((from_dev(st, np.uint32), from_dev(out, dtype(kb)), from_dev(qst, np.uint32),
+                    from_dev(b.status_words(), np.uint32)))
+    (s1, o1, q1, w1), (s2, o2, q2, w2) = res
+    for t in range(len(caps)):
+        s = slice(int(off[t]), int(off[t + 1]))
+        np.testing.assert_array_equal(np.sort(s1[s]), np.sort(s2[s]), err_msg=f"table {t}")
+    assert set(np.unique(s1)) >= {S_NONE, S_DUP, S_INVALID_KEY, S_PLE}
+    np.testing.assert_array_equal(w1, w2)
+    # which keys an overfilled table keeps depends on the insert order: the
+    # lookups are compared on the other tables
+    for t in range(len(caps)):
+        if t in overfilled:
+            continue
+        q = slice(int(qoff[t]), int(qoff[t + 1]))
+        np.testing.assert_array_equal(q1[q], q2[q], err_msg=f"table {t}")
+        np.testing.assert_array_equal(o1[q], o2[q], err_msg=f"table {t}")
+
+
 @pytest.mark.parametrize("kb,shape", [(32, (16, 16)), (32, (4, 8)), (32, (8, 16)), (64, (4, 4))])
 def test_table_batch_full_tables_and_invalid_keys(cuda, oracle, kb, shape):
     """Tables filled past capacity (every key beyond it reports

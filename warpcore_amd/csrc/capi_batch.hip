@@ -110,6 +110,14 @@ int write_report(const Batch<K, V>* b, uint64_t* rows, uint64_t max_rows, double
         const detail::BatchLaunch l{item_keys, int(blocks_per_cu), int(block_size)};                             \
         return static_cast<Batch_##NAME*>(h)->b.set_launch(l) ? 0 : wc_capi::kBadArgument;                        \
     }                                                                                                             \
+    extern "C" int wc_batch_##NAME##_set_lds_insert(void* h, int on) {                                            \
+        if (!h) return wc_capi::kBadHandle;                                                                       \
+        static_cast<Batch_##NAME*>(h)->b.set_lds_insert(on != 0);                                                 \
+        return 0;                                                                                                 \
+    }                                                                                                             \
+    extern "C" uint64_t wc_batch_##NAME##_lds_tables(void* h) {                                                   \
+        return h ? static_cast<Batch_##NAME*>(h)->b.lds_tables() : 0;                                             \
+    }                                                                                                             \
     extern "C" int wc_batch_##NAME##_get_launch(void* h, uint64_t* launch_out) {                                  \
         if (!h || !launch_out) return wc_capi::kBadHandle;                                                        \
         const auto& l = static_cast<Batch_##NAME*>(h)->b.launch();                                                \

@@ -92,6 +92,10 @@ WC_H index_t batch_table_windows(index_t min_capacity, index_t window) noexcept 
     return next_prime_at_least(w < 2 ? 2 : w);
 }
 
+// Tables of at most this many bytes of slots are inserted in LDS
+// (batch_lds_insert): two blocks per CU.
+constexpr index_t kBatchLdsBytes = 64 * 1024;
+
 struct BatchLaunch {
     index_t item_keys;   // keys per work item
     int blocks_per_cu;   // persistent grid: CUs * blocks_per_cu blocks
@@ -141,15 +145,16 @@ WC_D void batch_dispatch(unsigned config, F& f) {
 
 namespace kernels {
 
-// items[i] = SDIV(ends[t] - begins[t], item_keys) for t = order[i]
+// items[i] = SDIV(ends[t] - begins[t], item_keys) for t = order[i], or 0
+// from position `limit` on (tables handled elsewhere)
 template<class T>
 __global__ __launch_bounds__(256) void batch_count_items(const T* __restrict__ begins, const T* __restrict__ ends,
                                                          const std::uint32_t* __restrict__ order, index_t n,
-                                                         T item_keys, T* __restrict__ items) {
+                                                         T item_keys, T* __restrict__ items, index_t limit) {
     const index_t stride = index_t(gridDim.x) * blockDim.x;
     for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
         const std::uint32_t t = order[i];
-        items[i] = SDIV(ends[t] - begins[t], item_keys);
+        items[i] = i < limit ? SDIV(ends[t] - begins[t], item_keys) : T(0);
     }
 }
 
@@ -160,9 +165,9 @@ __global__ __launch_bounds__(256) void batch_fill_items(const T* __restrict__ fi
                                                         const T* __restrict__ ends,
                                                         const std::uint32_t* __restrict__ order, index_t n,
                                                         T item_keys, std::uint32_t* __restrict__ item_table,
-                                                        T* __restrict__ item_begin) {
+                                                        T* __restrict__ item_begin, index_t limit) {
     const index_t stride = index_t(gridDim.x) * blockDim.x;
-    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < limit && i < n; i += stride) {
         const std::uint32_t t = order[i];
         const T b = begins[t];
         const T m = SDIV(ends[t] - b, item_keys);
@@ -170,6 +175,125 @@ __global__ __launch_bounds__(256) void batch_fill_items(const T* __restrict__ fi
             item_table[first[i] + j] = t;
             item_begin[first[i] + j] = b + j * item_keys;
         }
+    }
+}
+
+// Inserts into the batch's smallest tables (positions [first, n) of the
+// table order, each table at most kBatchLdsBytes of slots), one block per
+// table at a time: the block copies the table's slots into LDS, inserts the
+// table's keys there with the per-key protocol of the global kernels
+// (windows in probe order; the key found in a window is a duplicate whose
+// value is overwritten, else the window's lowest EMPTY slot is claimed by a
+// compare-and-swap, and a lost swap rescans the window; a window with
+// neither sends the key on, up to the probing budget) and writes the slots
+// back. Statuses and the per-table status words are those of the global
+// kernels; only slot placement may differ.
+template<class Key, class Value, Key EmptyKey, Key TombstoneKey, class StatusHandler>
+__global__ __launch_bounds__(1024) void batch_lds_insert(
+    const Key* __restrict__ keys_in, const Value* __restrict__ values_in, const index_t* __restrict__ begins,
+    const index_t* __restrict__ ends, const std::uint32_t* __restrict__ order, index_t first, index_t n,
+    const detail::BatchTableDesc* __restrict__ tables, storage::key_value::pair_t<Key, Value>* slots,
+    Status* __restrict__ statuses, Key seed, index_t probing_length,
+    typename StatusHandler::base_type* __restrict__ status_out) {
+    using Slot = storage::key_value::pair_t<Key, Value>;
+    using Configs = detail::BatchConfigs<sizeof(Slot)>;
+    constexpr bool packed = sizeof(Slot) == 8;
+    static_assert(sizeof(Slot) == 8 || sizeof(Slot) == 16, "8- or 16-byte slots");
+    __shared__ Slot lslots[detail::kBatchLdsBytes / sizeof(Slot)];
+    for (index_t i = first + blockIdx.x; i < n; i += gridDim.x) {
+        const std::uint32_t t = order[i];
+        const detail::BatchTableDesc d = tables[t];
+        const index_t cap = d.geo.capacity;
+        const index_t b = begins[t], e = ends[t];
+        if (b >= e) continue;   // uniform over the block
+        Slot* g = slots + d.slot_offset;
+        for (index_t s = threadIdx.x; s < cap; s += blockDim.x) lslots[s] = g[s];
+        __syncthreads();
+        detail::DeferredJoin join;
+        auto run = [&](auto ic) {
+            constexpr unsigned WIN = Configs::window[decltype(ic)::value];
+            using Cursor = detail::WindowCursor<defaults::probing_scheme_t<Key, WIN>>;
+            const index_t budget = probing_schemes::detail::window_budget(probing_length, WIN, d.geo.windows);
+            for (index_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+                const Key key = keys_in[k];
+                const Value val = values_in[k];
+                Status st = Status::probing_length_exceeded();
+                if (key == EmptyKey || key == TombstoneKey) {
+                    st = Status::invalid_key();
+                } else {
+                    Cursor c;
+                    c.begin(key, seed, d.geo);
+                    index_t w = c.w;
+                    for (index_t r = 0; r < budget; ++r) {
+                        Slot* win = lslots + w * WIN;
+                        bool done = false, next = false;
+                        while (!done && !next) {
+                            // the window's keys, all read before any is compared (other
+                            // threads' swaps change them: read again on every pass)
+                            __asm__ __volatile__("" ::: "memory");
+                            Key wk[WIN];
+                            std::uint64_t wword[packed ? WIN : 1];
+#pragma unroll
+                            for (unsigned j = 0; j < WIN; ++j) {
+                                if constexpr (packed) {
+                                    wword[j] = reinterpret_cast<const std::uint64_t*>(win)[j];
+                                    __builtin_memcpy(&wk[j], &wword[j], sizeof(Key));
+                                } else {
+                                    wk[j] = win[j].key;
+                                }
+                            }
+                            int empty_at = -1, found_at = -1;
+#pragma unroll
+                            for (unsigned j = 0; j < WIN; ++j) {
+                                if (found_at < 0 && wk[j] == key) found_at = int(j);
+                                if (empty_at < 0 && wk[j] == EmptyKey) empty_at = int(j);
+                            }
+                            if (found_at >= 0) {   // the key is stored: its value is overwritten
+                                win[found_at].value = val;
+                                st = Status::duplicate_key();
+                                done = true;
+                            } else if (empty_at < 0) {
+                                next = true;
+                            } else if constexpr (packed) {
+                                Slot want;
+                                want.key = key;
+                                want.value = val;
+                                std::uint64_t desired, expected = 0;
+                                __builtin_memcpy(&desired, &want, 8);
+#pragma unroll
+                                for (unsigned j = 0; j < WIN; ++j)
+                                    if (int(j) == empty_at) expected = wword[j];
+                                if (__hip_atomic_compare_exchange_strong(
+                                        reinterpret_cast<std::uint64_t*>(win + empty_at), &expected, desired,
+                                        __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                                    st = Status::none();
+                                    done = true;
+                                }
+                            } else {
+                                Key expected = EmptyKey;
+                                if (__hip_atomic_compare_exchange_strong(&win[empty_at].key, &expected, key,
+                                                                         __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                                         __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                                    win[empty_at].value = val;
+                                    st = Status::none();
+                                    done = true;
+                                }
+                            }
+                        }
+                        if (done) break;
+                        w += c.step;
+                        if (w >= d.geo.windows) w -= d.geo.windows;
+                    }
+                }
+                if (st.has_any()) join(st);
+                StatusHandler::handle(st, status_out, k);
+            }
+        };
+        detail::batch_dispatch<0, Configs::count>(d.config, run);
+        join.flush(statuses + t);
+        __syncthreads();
+        for (index_t s = threadIdx.x; s < cap; s += blockDim.x) g[s] = lslots[s];
+        __syncthreads();
     }
 }
 
@@ -262,6 +386,11 @@ public:
               malloc_(&items_, sizeof(index_type) * num_tables) &&
               malloc_(&first_item_, sizeof(index_type) * num_tables);
         if (ok_) ok_ = set_config(config);
+        // the LDS insert runs on a stream of its own beside the work items
+        if (ok_)
+            ok_ = hipStreamCreateWithFlags(&side_, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&fork_, hipEventDisableTiming) == hipSuccess &&
+                  hipEventCreateWithFlags(&join_, hipEventDisableTiming) == hipSuccess;
     }
     TableBatch(const TableBatch&) = delete;
     TableBatch& operator=(const TableBatch&) = delete;
@@ -269,6 +398,9 @@ public:
         for (void* p : {(void*)tables_, (void*)slots_, (void*)statuses_, (void*)order_, (void*)parts_,
                         (void*)counter_, (void*)items_, (void*)first_item_, (void*)item_table_, (void*)item_begin_})
             if (p) (void)hipFree(p);
+        if (join_) (void)hipEventDestroy(join_);
+        if (fork_) (void)hipEventDestroy(fork_);
+        if (side_) (void)hipStreamDestroy(side_);
     }
 
     WC_H bool ok() const noexcept { return ok_; }
@@ -322,6 +454,10 @@ public:
                 hipSuccess ||
             hipMemcpy(order_, order.data(), sizeof(std::uint32_t) * num_tables_, hipMemcpyHostToDevice) != hipSuccess)
             return false;
+        // the order's suffix of tables small enough for the LDS insert
+        lds_first_ = num_tables_;
+        while (lds_first_ > 0 && caps[order[lds_first_ - 1]] * sizeof(slot_type) <= detail::kBatchLdsBytes)
+            --lds_first_;
         capacities_ = std::move(caps);
         config_of_.assign(per_table, per_table + num_tables_);
         total_slots_ = slots;
@@ -372,6 +508,12 @@ public:
                   tune_report_type* report = nullptr) noexcept;
 
     WC_H const launch_type& launch() const noexcept { return launch_; }
+    // Inserts into tables of at most kBatchLdsBytes of slots run in LDS
+    // (on by default); off: every table through the work items.
+    WC_H void set_lds_insert(bool on) noexcept { lds_insert_ = on; }
+    WC_H bool lds_insert() const noexcept { return lds_insert_; }
+    // tables the LDS insert takes (the smallest ones)
+    WC_H index_type lds_tables() const noexcept { return num_tables_ - lds_first_; }
     WC_H bool set_launch(const launch_type& l) noexcept {
         if (l.item_keys < 1 || l.blocks_per_cu < 1 || l.block_size < 64 || l.block_size > 512 ||
             l.block_size % 64 != 0)
@@ -420,16 +562,30 @@ private:
             }
             item_capacity_ = max_items;
         }
+        // inserts into the smallest tables run in LDS; the work items cover the others
+        const index_type limit = IsInsert && lds_insert_ ? lds_first_ : num_tables_;
+        if (limit < num_tables_) {   // on the side stream, after what the caller's stream holds
+            const index_type n = num_tables_ - limit;
+            const index_type cap = index_type(detail::num_compute_units()) * 2;
+            (void)hipEventRecord(fork_, stream);
+            (void)hipStreamWaitEvent(side_, fork_, 0);
+            kernels::batch_lds_insert<key_type, value_type, EmptyKey, TombstoneKey, StatusHandler>
+                <<<unsigned(n < cap ? n : cap), 1024, 0, side_>>>(keys_in, values_in, begins, ends, order_, limit,
+                                                                  num_tables_, tables_, slots_, statuses_, seed_,
+                                                                  probing_length, status_out);
+            (void)hipEventRecord(join_, side_);
+        }
         kernels::batch_count_items<index_type><<<detail::grid_for(num_tables_, 1, 8), block_size(), 0, stream>>>(
-            begins, ends, order_, num_tables_, ik, items_);
+            begins, ends, order_, num_tables_, ik, items_, limit);
         detail::device_scan(items_, first_item_, num_tables_, parts_, false, stream);
         kernels::batch_fill_items<index_type><<<detail::grid_for(num_tables_, 1, 8), block_size(), 0, stream>>>(
-            first_item_, begins, ends, order_, num_tables_, ik, item_table_, item_begin_);
+            first_item_, begins, ends, order_, num_tables_, ik, item_table_, item_begin_, limit);
         storage::kernels::fill<index_type><<<1, 64, 0, stream>>>(counter_, 1, index_type(0));
         const unsigned grid = unsigned(detail::num_compute_units() * launch.blocks_per_cu);
         launch_owner<StatusHandler, IsInsert, -1>(single_config_, grid, unsigned(launch.block_size), keys_in,
                                                   values_in, values_out, ends, ik, stream, probing_length,
                                                   status_out);
+        if (limit < num_tables_) (void)hipStreamWaitEvent(stream, join_, 0);   // the caller's stream holds both
         last_error_ = int(hipGetLastError());
     }
 
@@ -487,6 +643,10 @@ private:
     index_type total_slots_ = 0;
     index_type arena_slots_ = 0;
     launch_type launch_ = default_launch();
+    index_type lds_first_ = 0;   // order positions from here on: tables small enough for the LDS insert
+    bool lds_insert_ = true;
+    hipStream_t side_ = nullptr;
+    hipEvent_t fork_ = nullptr, join_ = nullptr;
     bool ok_ = false;
     int last_error_ = 0;
     detail::BatchTableDesc* tables_ = nullptr;

@@ -672,6 +672,15 @@ class TableBatch:
     def set_launch(self, item_keys: int, blocks_per_cu: int, block_size: int = 256):
         _check(self._fn("set_launch")(self._h, item_keys, blocks_per_cu, block_size), "set_launch")
 
+    def set_lds_insert(self, on: bool):
+        """Inserts into tables of at most 64 KiB of slots run in LDS, one
+        block per table (default), or through the work items (False)."""
+        _check(self._fn("set_lds_insert")(self._h, int(bool(on))), "set_lds_insert")
+
+    def lds_tables(self) -> int:
+        """how many tables the LDS insert takes"""
+        return self._fn("lds_tables")(self._h)
+
     def get_launch(self):
         """(keys per work item, blocks per CU, threads per block)"""
         out = (ctypes.c_uint64 * 3)()
