@@ -536,10 +536,11 @@ WC_API uint32_t wc_bucketlist_u64u64_cg_size(void* table);
  *                key results and statuses as SingleValueHashTable's
  *   set_launch / get_launch  keys per work item, blocks per CU, threads per
  *                block (64..512, a multiple of 64)
- *   set_lds_insert  inserts into the tables of at most 64 KiB of slots run
- *                in LDS, one block per table (1, the default) or through the
- *                work items like the others (0); same results and statuses
- *   lds_tables   how many tables the LDS insert takes
+ *   set_lds_insert  inserts and lookups in the tables of at most 64 KiB of
+ *                slots run in LDS, one block per table, beside the work items
+ *                (1, the default) or through the work items like the others
+ *                (0); same results and statuses
+ *   lds_tables   how many tables the LDS path takes
  *   tune         autotuner: tables are put in classes by requested capacity
  *                (bands [4^k, 4^(k+1))) and fill (keys / requested capacity:
  *                < 0.6, < 0.7, < 0.8, >= 0.8); every configuration is timed

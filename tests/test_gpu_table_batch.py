@@ -119,8 +119,8 @@ def test_table_batch_matches_plain_launches(cuda):
 
 @pytest.mark.parametrize("kb", [32, 64])
 def test_table_batch_lds_insert_matches_work_items(cuda, kb):
-    """The LDS insert of the small tables against the work-item kernel on
-    the same batch: per-table insert statuses (sorted: which occurrence of a
+    """The LDS inserts and lookups of the small tables against the
+    work-item kernel on the same batch: per-table insert statuses (sorted: which occurrence of a
     repeated key gets none is open), every lookup result and status, and the
     per-table status words are equal; some tables are filled past capacity
     and some keys are invalid."""

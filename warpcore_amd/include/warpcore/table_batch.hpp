@@ -297,6 +297,79 @@ __global__ __launch_bounds__(1024) void batch_lds_insert(
     }
 }
 
+// Lookups in the same small tables, one block per table: slots copied into
+// LDS, each key's windows read there in probe order (a hit writes the value;
+// a window with an EMPTY slot ends the key as missing, its output left
+// alone; the probing budget as in the global kernels). Statuses and status
+// words are those of the global lookups.
+template<class Key, class Value, Key EmptyKey, Key TombstoneKey, class StatusHandler>
+__global__ __launch_bounds__(1024) void batch_lds_retrieve(
+    const Key* __restrict__ keys_in, Value* __restrict__ values_out, const index_t* __restrict__ begins,
+    const index_t* __restrict__ ends, const std::uint32_t* __restrict__ order, index_t first, index_t n,
+    const detail::BatchTableDesc* __restrict__ tables, const storage::key_value::pair_t<Key, Value>* slots,
+    Status* __restrict__ statuses, Key seed, index_t probing_length,
+    typename StatusHandler::base_type* __restrict__ status_out) {
+    using Slot = storage::key_value::pair_t<Key, Value>;
+    using Configs = detail::BatchConfigs<sizeof(Slot)>;
+    __shared__ Slot lslots[detail::kBatchLdsBytes / sizeof(Slot)];
+    for (index_t i = first + blockIdx.x; i < n; i += gridDim.x) {
+        const std::uint32_t t = order[i];
+        const detail::BatchTableDesc d = tables[t];
+        const index_t cap = d.geo.capacity;
+        const index_t b = begins[t], e = ends[t];
+        if (b >= e) continue;   // uniform over the block
+        const Slot* g = slots + d.slot_offset;
+        for (index_t s = threadIdx.x; s < cap; s += blockDim.x) lslots[s] = g[s];
+        __syncthreads();
+        detail::DeferredJoin join;
+        auto run = [&](auto ic) {
+            constexpr unsigned WIN = Configs::window[decltype(ic)::value];
+            using Cursor = detail::WindowCursor<defaults::probing_scheme_t<Key, WIN>>;
+            const index_t budget = probing_schemes::detail::window_budget(probing_length, WIN, d.geo.windows);
+            for (index_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+                const Key key = keys_in[k];
+                Status st = Status::probing_length_exceeded();
+                if (key == EmptyKey || key == TombstoneKey) {
+                    st = Status::invalid_key();
+                } else {
+                    Cursor c;
+                    c.begin(key, seed, d.geo);
+                    index_t w = c.w;
+                    for (index_t r = 0; r < budget; ++r) {
+                        const Slot* win = lslots + w * WIN;
+                        Key wk[WIN];
+#pragma unroll
+                        for (unsigned j = 0; j < WIN; ++j) wk[j] = win[j].key;
+                        int found_at = -1;
+                        bool empty = false;
+#pragma unroll
+                        for (unsigned j = 0; j < WIN; ++j) {
+                            if (found_at < 0 && wk[j] == key) found_at = int(j);
+                            empty = empty || wk[j] == EmptyKey;
+                        }
+                        if (found_at >= 0) {
+                            values_out[k] = win[found_at].value;
+                            st = Status::none();
+                            break;
+                        }
+                        if (empty) {
+                            st = Status::key_not_found();
+                            break;
+                        }
+                        w += c.step;
+                        if (w >= d.geo.windows) w -= d.geo.windows;
+                    }
+                }
+                if (st.has_any()) join(st);
+                StatusHandler::handle(st, status_out, k);
+            }
+        };
+        detail::batch_dispatch<0, Configs::count>(d.config, run);
+        join.flush(statuses + t);
+        __syncthreads();
+    }
+}
+
 // Persistent lane-owner kernel over the work items: each wavefront takes
 // items until none is left. Only >= 0: every table has that configuration.
 template<class Key, class Value, Key EmptyKey, Key TombstoneKey, class StatusHandler, bool IsInsert, int Only>
@@ -508,8 +581,8 @@ public:
                   tune_report_type* report = nullptr) noexcept;
 
     WC_H const launch_type& launch() const noexcept { return launch_; }
-    // Inserts into tables of at most kBatchLdsBytes of slots run in LDS
-    // (on by default); off: every table through the work items.
+    // Inserts and lookups in tables of at most kBatchLdsBytes of slots run
+    // in LDS (on by default); off: every table through the work items.
     WC_H void set_lds_insert(bool on) noexcept { lds_insert_ = on; }
     WC_H bool lds_insert() const noexcept { return lds_insert_; }
     // tables the LDS insert takes (the smallest ones)
@@ -562,17 +635,22 @@ private:
             }
             item_capacity_ = max_items;
         }
-        // inserts into the smallest tables run in LDS; the work items cover the others
-        const index_type limit = IsInsert && lds_insert_ ? lds_first_ : num_tables_;
+        // the smallest tables run in LDS; the work items cover the others
+        const index_type limit = lds_insert_ ? lds_first_ : num_tables_;
         if (limit < num_tables_) {   // on the side stream, after what the caller's stream holds
             const index_type n = num_tables_ - limit;
             const index_type cap = index_type(detail::num_compute_units()) * 2;
+            const unsigned grid = unsigned(n < cap ? n : cap);
             (void)hipEventRecord(fork_, stream);
             (void)hipStreamWaitEvent(side_, fork_, 0);
-            kernels::batch_lds_insert<key_type, value_type, EmptyKey, TombstoneKey, StatusHandler>
-                <<<unsigned(n < cap ? n : cap), 1024, 0, side_>>>(keys_in, values_in, begins, ends, order_, limit,
-                                                                  num_tables_, tables_, slots_, statuses_, seed_,
-                                                                  probing_length, status_out);
+            if constexpr (IsInsert)
+                kernels::batch_lds_insert<key_type, value_type, EmptyKey, TombstoneKey, StatusHandler>
+                    <<<grid, 1024, 0, side_>>>(keys_in, values_in, begins, ends, order_, limit, num_tables_, tables_,
+                                               slots_, statuses_, seed_, probing_length, status_out);
+            else
+                kernels::batch_lds_retrieve<key_type, value_type, EmptyKey, TombstoneKey, StatusHandler>
+                    <<<grid, 1024, 0, side_>>>(keys_in, values_out, begins, ends, order_, limit, num_tables_,
+                                               tables_, slots_, statuses_, seed_, probing_length, status_out);
             (void)hipEventRecord(join_, side_);
         }
         kernels::batch_count_items<index_type><<<detail::grid_for(num_tables_, 1, 8), block_size(), 0, stream>>>(
