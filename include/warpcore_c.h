@@ -536,7 +536,7 @@ WC_API uint32_t wc_bucketlist_u64u64_cg_size(void* table);
  *                key results and statuses as SingleValueHashTable's
  *   set_launch / get_launch  keys per work item, blocks per CU, threads per
  *                block (64..512, a multiple of 64)
- *   set_lds_insert  inserts and lookups in the tables of at most 64 KiB of
+ *   set_lds_insert  inserts and lookups in the tables of at most 128 KiB of
  *                slots run in LDS, one block per table, beside the work items
  *                (1, the default) or through the work items like the others
  *                (0); same results and statuses

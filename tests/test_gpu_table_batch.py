@@ -58,17 +58,22 @@ def _check_against_oracle(oracle, b, kb, caps, keys, vals, queries, qoff, st, go
 
 # every configuration of each key width (index into TableBatch.configs()),
 # and "mixed": a random configuration per table
-CONFIG_CASES = [(32, c) for c in range(5)] + [(64, c) for c in range(3)] + [(32, "mixed"), (64, "mixed")]
+# ("mixed-items": mixed, with every table through the work items instead of
+# the LDS path the small tables take by default)
+CONFIG_CASES = ([(32, c) for c in range(5)] + [(64, c) for c in range(3)] + [(32, "mixed"), (64, "mixed")] +
+                [(32, "mixed-items"), (64, "mixed-items")])
 
 
 @pytest.mark.parametrize("kb,config", CONFIG_CASES)
 def test_table_batch_matches_oracle_per_table(cuda, oracle, kb, config):
     from warpcore_amd.tables import TableBatch
     print(f"[batch] kb={kb} config={config}", flush=True)
-    caps, keys, vals, queries, off, qoff = _workload(600, kb, seed=3 + (17 if config == "mixed" else config))
+    mixed = config in ("mixed", "mixed-items")
+    caps, keys, vals, queries, off, qoff = _workload(600, kb, seed=3 + (17 if mixed else config))
     b = TableBatch(caps, kb, kb)
+    b.set_lds_insert(config != "mixed-items")
     n_cfg = len(b.configs())
-    if config == "mixed":
+    if mixed:
         b.set_configs(np.random.default_rng(5).integers(0, n_cfg, len(caps)))
     else:
         b.set_configs(np.full(len(caps), config))

@@ -673,7 +673,7 @@ class TableBatch:
         _check(self._fn("set_launch")(self._h, item_keys, blocks_per_cu, block_size), "set_launch")
 
     def set_lds_insert(self, on: bool):
-        """Inserts and lookups in tables of at most 64 KiB of slots run in
+        """Inserts and lookups in tables of at most 128 KiB of slots run in
         LDS, one block per table (default), or through the work items
         (False)."""
         _check(self._fn("set_lds_insert")(self._h, int(bool(on))), "set_lds_insert")
