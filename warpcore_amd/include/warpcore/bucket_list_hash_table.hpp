@@ -560,15 +560,45 @@ struct BucketListView {
 };
 
 // Key pass of a bucket-list insert on the lane-owner kernel: finds or
-// claims each key's slot in the key table without touching its handle and
-// records the slot of every input pair (slot_of; it stays ~0 where the key
-// failed) for the append pass.
-template<class KeyView>
+// claims each key's slot in the key table and records the slot of every
+// input pair (slot_of; it stays ~0 where the key failed) for the append
+// pass. The pair whose compare-and-swap claims a new key also makes that
+// key's first append here: its first bucket (min_size cells, from the
+// wavefront's arena) gets the pair's value and the handle becomes
+// {initialized, bucket, 1, 1}, as the append's uninitialised path leaves
+// it; the handle is a plain store, read by the append pass only (a later
+// launch), and the pair is marked kSlotAppended. If no arena holds the
+// bucket the handle becomes {full, head, 0, 0}, again as that path leaves
+// it, and the pair is marked kSlotAppendFailed (the append pass reports its
+// out_of_memory).
+constexpr std::uint32_t kSlotAppended = ~1u;       // slot_of marks: appended by the key pass,
+constexpr std::uint32_t kSlotAppendFailed = ~2u;   // or its allocation failed there
+
+template<class KeyView, class StoreView>
 struct BucketListKeyCore : KeyView {
     std::uint32_t* slot_of;
+    const typename StoreView::value_type* values;   // the pairs' values, by position
+    StoreView store;
+    index_t max_values;
     template<class Join>
     WC_D void finish_insert(index_t, const typename KeyView::value_type&, bool, Status&, Join&) const noexcept {}
-    WC_D void on_slot(index_t slot, index_t pos) const noexcept { slot_of[pos] = std::uint32_t(slot); }
+    WC_D void on_slot(index_t slot, index_t pos, bool claimed) const noexcept {
+        if (claimed && max_values > 0) {
+            using LS = storage::multi_value::detail::LinkedListState;
+            using handle_type = typename KeyView::value_type;
+            const index_t at = store.allocate(store.min_size);
+            if (at != ~index_t(0)) {
+                store.buckets[at].value = values[pos];
+                this->slots[slot].value = handle_type::make(LS::initialized, at, 1, 1);
+                slot_of[pos] = kSlotAppended;
+            } else {
+                this->slots[slot].value = handle_type::make(LS::full, StoreView::head(), 0, 0);
+                slot_of[pos] = kSlotAppendFailed;
+            }
+            return;
+        }
+        slot_of[pos] = std::uint32_t(slot);
+    }
 };
 
 }  // namespace detail
@@ -591,7 +621,10 @@ __global__ __launch_bounds__(256) void bl_append(const typename View::value_type
     for (index_t i = index_t(blockIdx.x) * blockDim.x + threadIdx.x; i < num_in; i += stride) {
         const std::uint32_t slot = slot_of[i];
         Status st = key_status != nullptr ? key_status[i] : Status::none();
-        if (slot != ~0u) {
+        if (slot == detail::kSlotAppendFailed) {
+            join(Status::out_of_memory());
+            st = st + Status::out_of_memory();
+        } else if (slot < detail::kSlotAppendFailed) {   // not failed, not appended by the key pass
             handle_type* h = &view.keys.slots[slot].value;
             const std::uint64_t seen = warpcore::detail::atomic_load_relaxed(&h->pack);
             Status a = Status::max_values_for_key_reached();
@@ -847,14 +880,16 @@ public:
         static_assert(checks::is_status_handler<StatusHandler>(), "not a valid status handler type");
         if (!hash_table_.is_initialized_ || num_in == 0) return;
         using key_view = typename hash_table_type::view_type;
-        using key_core = detail::BucketListKeyCore<key_view>;
+        using key_core = detail::BucketListKeyCore<key_view, typename value_store_type::view_type>;
         if constexpr (warpcore::detail::fixed_step_scheme<typename key_view::probing_scheme_type>::value &&
                       warpcore::detail::owner_insert_capable<key_core>()) {
             // key pass on the lane-owner kernel, then one lane per pair appends
             // (in chunks of the table's scratch: each pair's slot and key status)
             constexpr index_type chunk = index_type(1) << 24;
             const index_type cap = num_in < chunk ? num_in : chunk;
-            if (warpcore::detail::owner_inserts() && reserve_scratch(cap)) {
+            // (slot indices below detail::kSlotAppendFailed fit slot_of)
+            if (warpcore::detail::owner_inserts() && hash_table_.capacity() < index_type(detail::kSlotAppendFailed) &&
+                reserve_scratch(cap)) {
                 constexpr bool statuses = !std::is_same<StatusHandler, status_handlers::ReturnNothing>::value;
                 std::uint32_t* slot_of = scratch_;
                 Status* key_status = reinterpret_cast<Status*>(scratch_ + scratch_cap_);
@@ -862,7 +897,8 @@ public:
                     const index_type n = num_in - first < cap ? num_in - first : cap;
                     storage::kernels::fill<std::uint32_t>
                         <<<warpcore::detail::grid_for(n, 1, 8), block_size(), 0, stream>>>(slot_of, n, ~0u);
-                    const key_core core{hash_table_.view(), slot_of};
+                    const key_core core{hash_table_.view(), slot_of, values_in + first, value_store_.view(),
+                                        max_values_per_key_};
                     Status* ks = statuses && status_out != nullptr ? key_status : nullptr;
                     if (ks != nullptr)
                         warpcore::detail::launch_insert_owner<key_core, status_handlers::ReturnStatus, false>(

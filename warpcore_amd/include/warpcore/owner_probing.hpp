@@ -59,15 +59,16 @@ WC_D std::uint64_t group_field(std::uint64_t m, unsigned shift) noexcept {
 }
 
 // Cores that want each key's slot (found or claimed) by input position
-// provide on_slot(slot, position).
+// provide on_slot(slot, position, claimed): claimed when this key's own
+// compare-and-swap took the slot.
 template<class C, class = void>
 struct has_slot_hook : std::false_type {};
 template<class C>
-struct has_slot_hook<C, std::void_t<decltype(std::declval<const C&>().on_slot(index_t(0), index_t(0)))>>
+struct has_slot_hook<C, std::void_t<decltype(std::declval<const C&>().on_slot(index_t(0), index_t(0), false))>>
     : std::true_type {};
 template<class Core>
-WC_D void record_slot(const Core& core, index_t slot, index_t pos) noexcept {
-    if constexpr (has_slot_hook<Core>::value) core.on_slot(slot, pos);
+WC_D void record_slot(const Core& core, index_t slot, index_t pos, bool claimed) noexcept {
+    if constexpr (has_slot_hook<Core>::value) core.on_slot(slot, pos, claimed);
 }
 
 // Slot in its window of window-mask bit b (groups of G lanes, SPL slots each).
@@ -259,7 +260,7 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
                 if (code == 1 || code == 2) {   // claimed, or the key was already there
                     Status st = code == 1 ? Status::none() : Status::duplicate_key();
                     core.finish_insert(index_t(w) * WIN + lead, val, code == 1 && Core::publishes_value, st, join);
-                    record_slot(core, index_t(w) * WIN + lead, pos);
+                    record_slot(core, index_t(w) * WIN + lead, pos, code == 1);
                     if (code == 2) join(Status::duplicate_key());
                     StatusHandler::handle(st, status_out, pos);
                     phase = kFree;
@@ -311,7 +312,7 @@ WC_D void insert_owner_range(const typename Core::key_type* __restrict__ keys_in
                     Status st = Status::duplicate_key();
                     const index_t at = index_t(w) * WIN + owner_slot_of_bit<CG, SPL>(unsigned(ffs64(std::uint64_t(hit)) - 1));
                     core.finish_insert(at, val, false, st, join);
-                    record_slot(core, at, pos);
+                    record_slot(core, at, pos, false);
                     join(Status::duplicate_key());
                     StatusHandler::handle(st, status_out, pos);
                     phase = kFree;
