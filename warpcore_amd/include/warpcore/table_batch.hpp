@@ -470,6 +470,7 @@ public:
     TableBatch(const TableBatch&) = delete;
     TableBatch& operator=(const TableBatch&) = delete;
     WC_H ~TableBatch() noexcept {
+        if (side_) (void)hipStreamSynchronize(side_);   // its LDS kernels read and write the arena
         for (void* p : {(void*)tables_, (void*)slots_, (void*)statuses_, (void*)order_, (void*)parts_,
                         (void*)counter_, (void*)items_, (void*)first_item_, (void*)item_table_, (void*)item_begin_})
             if (p) (void)hipFree(p);
@@ -514,6 +515,7 @@ public:
             caps[t] = p * win;
             slots += p * win;
         }
+        if (side_) (void)hipStreamSynchronize(side_);   // LDS kernels of earlier calls use the arena
         if (slots > arena_slots_) {
             slot_type* grown = nullptr;
             if (!malloc_(&grown, sizeof(slot_type) * slots)) return false;
