@@ -59,7 +59,7 @@ def algorithmic_bytes(op: str, kb: int, vb: int) -> int:
 
 # Revision of the probing kernels' code; a traffic figure measured on
 # another revision is not used.
-KERNEL_REVISION = "owner-3"
+KERNEL_REVISION = "owner-4"
 
 
 def read_traffic(family: str, log2n: int = 28, table: str = "u64"):

@@ -438,8 +438,12 @@ WC_D void retrieve_owner_range(const typename Core::key_type* __restrict__ keys_
 #pragma unroll
         for (unsigned j = 0; j < CG; ++j) {
             const W ew = lds[shift + j].w;
-            const W at = ew != kNoWindow<W> ? ew : W(0);
-            owner_load_pair<SPL>(lane_slots + index_t(at) * WIN, s[j]);
+            if (ew != kNoWindow<W>) {   // no load for an owner without a window this round
+                owner_load_pair<SPL>(lane_slots + index_t(ew) * WIN, s[j]);
+            } else {
+#pragma unroll
+                for (unsigned x = 0; x < SPL; ++x) s[j][x] = Slot{};
+            }
         }
         std::uint32_t fword = 0;
         if (phase == kFilter) fword = filter[w >> 2];
