@@ -38,6 +38,35 @@ def combine(request):
     lib.wc_set_counting_combine(saved)
 
 
+@pytest.mark.parametrize("kb", [64, 32])
+def test_counting_status_word_without_status_output(cuda, combine, oracle, kb):
+    """Inserts with no per-occurrence statuses still join duplicate_key
+    into the table's status word exactly when the oracle does: not for a
+    batch of distinct new keys, yes for repeats inside a batch (combined
+    into one entry of several occurrences) and for keys already stored."""
+    from warpcore_amd.tables import CountingHashTable
+    cap = 1 << 15
+    t = CountingHashTable(cap, kb)
+    o = oracle.Counting(cap, kb)
+    fresh = unique_keys(5000, kb, seed=81)
+    t.insert(to_dev(fresh))
+    o.insert(fresh)
+    assert t.peek_status() == o.status() == S_NONE
+    more = unique_keys(3000, kb, seed=82, exclude=fresh)
+    rep = np.concatenate([more, more[:100]])          # 100 keys twice in one batch
+    t.insert(to_dev(rep))
+    o.insert(rep)
+    assert t.peek_status() == o.status() == S_DUP
+    t.pop_status()
+    assert t.peek_status() == S_NONE
+    again = fresh[:50]                                # stored keys, once each
+    t.insert(to_dev(again))
+    assert t.peek_status() == S_DUP
+    out = dev_empty(len(again), 64)
+    t.retrieve(to_dev(again), out)
+    np.testing.assert_array_equal(from_dev(out, np.uint64), np.full(len(again), 2, np.uint64))
+
+
 @pytest.mark.parametrize("kb,cg,s", [(64, 4, 1.1), (64, 8, 1.0), (32, 4, 1.1), (64, 16, 0.8), (64, 64, 1.1)])
 def test_counting_zipf_exact_counts(cuda, shape, combine, oracle, kb, cg, s):
     from warpcore_amd.tables import CountingHashTable
