@@ -206,6 +206,35 @@ def test_bucketlist_arenas_out_of_memory(cuda):
         assert len(set(got)) == len(got) and set(got) <= set(exp[int(keys[i])])
 
 
+def test_bucketlist_out_of_memory_on_first_buckets(cuda):
+    """A pool too small for every key's first bucket: a key whose first
+    bucket does not fit reports out_of_memory on all its pairs (its handle
+    is left full), a key that got one keeps its stored values, and no key
+    holds more than its own values."""
+    from warpcore_amd.tables import BucketListHashTable
+    keys, mult, k, v = _pairs(500, 32, 6.0, seed=17)
+    t = BucketListHashTable(2000, 300, 32, 32, grow_factor=1.5, min_bucket_size=2, max_bucket_size=32)
+    st = dev_empty(len(k), 32)
+    t.insert(to_dev(k), to_dev(v), status_out=st)
+    s = from_dev(st, np.uint32)
+    oom = (s & S_OOM) != 0
+    assert set(np.unique(s & ~np.uint32(S_OOM))) <= {S_NONE, S_DUP}
+    assert t.num_keys() == len(keys)
+    stored = np.sum(~oom)
+    assert t.num_values() == stored and t.value_cells_used() >= 300
+    total, b, e, vals, _ = _retrieve(t, keys, stored + 8, 32)
+    assert total == stored
+    exp = _expected(k, v)
+    empty_keys = 0
+    for key, lo, hi in zip(keys.tolist(), b, e):
+        got = vals[int(lo):int(hi)].tolist()
+        assert len(set(got)) == len(got) and set(got) <= set(exp[key])
+        n_ok = int(np.sum(~oom[k == key]))
+        assert len(got) == n_ok
+        empty_keys += n_ok == 0
+    assert 0 < empty_keys < len(keys)   # some keys got no first bucket, others did
+
+
 def test_bucketlist_max_values_per_key(cuda, oracle):
     from warpcore_amd.tables import BucketListHashTable
     keys, mult, k, v = _pairs(2000, 64, 6.0, seed=13)
